@@ -1,0 +1,178 @@
+"""Benchmark: one sparse FFT (outer_loop) at n=2^27, k=16000 on one MI355X.
+
+A step is one whole transform of a device-resident signal: comb restriction, permute+filter
+of every loop, bucket FFTs, cutoffs, reverse hash and estimation, ending when Node[I_F] is in
+host memory (the reference's timed region, outerLoop.cu:65-190, without its per-call
+mallocs).  The signal, filter and plan are prepared before the timed region.
+
+Prints one JSON line.  Extra objects: roofline (dominant kernel = permute+filter, algorithmic
+bytes / its event-timed duration), cpu_baseline (the oracle's sequential transform on a
+bounded sample of the same workload), accuracy (missed locations, MSE vs the FP64 truth).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-sfft_amd"))
+
+METRIC = ("ms per SFFT at n=2^27,k=16000 (FP32 vs FP64, MSE≤1e-4); % HBM peak")
+PUBLISHED_MS = 172.697          # README.md:102 (Titan Xp, same parameters, FP64)
+HBM_PEAK_GBS = 8000.0
+GATHER_GRANULE = 64             # bytes per random gather (HBM access granule)
+
+CONFIGS = {
+    # name: (q, k, Bcst, loops_loc, loops_thresh, loops_est, RLFC_cst, RLFC_loops, snr_db)
+    "cfg2": (27, 16000, 0.65, 2, 2, 7, 4096, 2, None),
+    "cfg1": (20, 1000, 1.40, 2, 2, 5, 128, 2, None),
+    "cfg3": (27, 40000, 0.7, 3, 2, 8, 32768, 2, 30.0),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="budget for the CPU baseline sample (0 disables it)")
+    ap.add_argument("--seed", type=int, default=0x5FF70002)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (the transform is not sharded); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import sfft_amd as S
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    q, k, Bcst, ll, lt, le, rc, rl, snr = CONFIGS[args.config]
+    prm = S.derive_params(q, k, Bcst, ll, lt, le, rc, rl)
+    n, B, w, L = prm["n"], prm["B"], prm["w"], prm["loops"]
+    fp32 = args.precision == "fp32"
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    log(f"[bench] {args.config} n={n} k={k} B={B} W={prm['W_RLFC']} w={w} loops={L} "
+        f"prec={args.precision}")
+
+    # ---- inputs (outside the timed region) ----
+    t = time.time()
+    locs, vals = S.signal_spectrum(n, k, args.seed)
+    x = S.synthesize(n, k, locs, vals, snr_db=snr or 0.0, noise_seed=args.seed + 1, fp32=fp32)
+    X = S.truth_spectrum(x)
+    log(f"[bench] signal + truth spectrum: {time.time() - t:.2f}s")
+    t = time.time()
+    ft, ff = S.make_filter(prm["lobefrac"], prm["tolerance"], n, prm["b_f"], w)
+    win = S.filter_window(ff, n, B)
+    del ff
+    if fp32:
+        ft = S.to_complex64(ft)
+    log(f"[bench] filter design (FP64, w={w}): {time.time() - t:.2f}s")
+    a, ai, offs = S.draw_permutations(n, L, prm["RLFC_loops"], prm["W_RLFC"], args.seed + 2)
+
+    plan = S.Plan(prm, args.precision)
+    plan.set_filter(ft, win)
+    plan.set_profiling(True)
+
+    # ---- warm-up, then K timed transforms (Node[I_F] host-visible at the end of each) ----
+    for _ in range(args.warmup):
+        nodes, cnt = plan.execute(x, a, ai, offs)
+    torch.cuda.synchronize()
+    stage_sum = None
+    step_ms = []
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        nodes, cnt = plan.execute(x, a, ai, offs)
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+        st = plan.stage_times()
+        stage_sum = st if stage_sum is None else {kk: stage_sum[kk] + v for kk, v in st.items()}
+    torch.cuda.synchronize()
+    total_s = time.perf_counter() - t_all
+    ms_per_step = total_s * 1e3 / args.steps
+    stages_ms = {kk: v * 1e3 / args.steps for kk, v in stage_sum.items()}
+    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (median "
+        f"{float(np.median(step_ms)):.3f}); stages(ms) "
+        + " ".join(f"{kk}={v:.3f}" for kk, v in stages_ms.items()))
+
+    acc = S.metric(nodes, k, locs, X)
+    log(f"[bench] I_F={cnt} missed={acc['missed_est']},{acc['missed_res']} "
+        f"MSE*k={acc['mse_k']:.4e} MSE={acc['mse']:.4e}")
+
+    # ---- roofline of the dominant kernel: permute+filter ----
+    e = 8 if fp32 else 16
+    pf_bytes = L * w * GATHER_GRANULE + w * e + L * B * e
+    pf_s = stages_ms["perm+filter"] * 1e-3
+    achieved = pf_bytes / pf_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get(f"perm_filter_{args.config}_{args.precision}")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "perm_filter_kernel", "bytes_per_launch": pf_bytes,
+                "launch_ms": round(stages_ms["perm+filter"], 4)}
+    # whole-transform algorithmic rate (SURVEY formula: gathers + taps + nodes)
+    sig = n // prm["W_RLFC"]
+    g_c = GATHER_GRANULE if sig * e >= GATHER_GRANULE else sig * e
+    x_bytes = min(prm["RLFC_loops"] * prm["W_RLFC"] * g_c + L * w * GATHER_GRANULE, n * e)
+    alg = x_bytes + w * e + cnt * (24 if not fp32 else 16)
+    transform = {"alg_bytes": alg, "GB/s": round(alg / (ms_per_step * 1e-3) / 1e9, 1),
+                 "frac": round(alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    # ---- CPU baseline: the oracle's sequential transform, 1 core, bounded sample ----
+    cpu = None
+    if args.cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        xh = x.cpu().numpy().astype(np.complex128)
+        fth = ft.cpu().numpy().astype(np.complex128)
+        times = []
+        t_budget = time.perf_counter()
+        while True:
+            t0 = time.perf_counter()
+            O.outer_loop64(prm, xh, fth, win, a, ai, offs)
+            times.append(time.perf_counter() - t0)
+            if time.perf_counter() - t_budget + times[-1] > args.cpu_seconds or len(times) >= 5:
+                break
+        cpu = {"value": round(1e3 * float(np.mean(times)), 1), "unit": "ms", "cores": 1,
+               "kind": "port",
+               "sample": f"{len(times)} full {args.config} transforms of the same signal, "
+                         f"filter and draws (oracle outer_loop64, gcc -O2, 1 thread)"}
+        log(f"[bench] cpu baseline: {cpu['value']} ms/transform over {len(times)} runs")
+
+    out = {
+        "metric": METRIC, "value": round(ms_per_step, 4), "unit": "ms", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": round(ms_per_step / PUBLISHED_MS, 6) if args.config == "cfg2" and
+        not fp32 else None,
+        "dtype": "f32" if fp32 else "f64", "data": "synthetic",
+        "config": {"workload": f"{args.config}: sfft n=2^{q} k={k} Bcst={Bcst} "
+                               f"loops={ll}+{le} thresh={lt} RLFC={rc}x{rl} "
+                               f"{'noise ' + str(snr) + 'dB' if snr else 'noiseless'}, "
+                               f"{args.precision} path",
+                   "n": n, "k": k, "B": B, "W_RLFC": prm["W_RLFC"], "w": w, "loops": L,
+                   "parallelism": "none (1 GPU)"},
+        "median_ms": round(float(np.median(step_ms)), 4),
+        "stages_ms": {kk: round(v, 4) for kk, v in stages_ms.items()},
+        "roofline": roofline, "transform_roofline": transform,
+        "accuracy": {"I_F": cnt, "missed": [acc["missed_est"], acc["missed_res"]],
+                     "mse_k": acc["mse_k"], "mse": acc["mse"], "max_abs": acc["max_abs"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,121 @@
+// sfft_internal.h -- shared declarations between the HIP kernels (sfft_kernels.hip) and the
+// host orchestration (sfft_plan.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/sfft.h"
+
+namespace sfft {
+
+// ---- canonical FFT description -----------------------------------------------------------
+// Pass split: log2N bits over p = ceil(log2N / 11) passes, the first (log2N mod p) passes
+// one bit larger.  Every table value comes from canon_twiddle().
+constexpr int kMaxPassLog = 11;
+constexpr int kMaxPasses = 4;
+
+int fft_plan_passes(int log2n, int* pass_logs);
+void canon_twiddle(uint64_t e, uint64_t T, int inverse, double* c, double* s);
+
+// Device twiddle tables of one FFT size/direction/precision.
+struct FftTables {
+  int log2n = -1;
+  int inverse = 0;
+  int is_fp32 = 0;
+  int npass = 0;
+  int logs[kMaxPasses] = {0, 0, 0, 0};
+  const void* twR[kMaxPasses] = {nullptr, nullptr, nullptr, nullptr};   // R entries
+  const void* tdir[kMaxPasses] = {nullptr, nullptr, nullptr, nullptr};  // Lcur entries
+  const void* thi[kMaxPasses] = {nullptr, nullptr, nullptr, nullptr};   // Lcur >> s
+  const void* tlo[kMaxPasses] = {nullptr, nullptr, nullptr, nullptr};   // 1 << s
+  int s[kMaxPasses] = {0, 0, 0, 0};
+  void* storage = nullptr;  // one device allocation holding every table
+  size_t bytes = 0;
+};
+
+hipError_t fft_tables_create(int log2n, int inverse, int is_fp32, FftTables* t);
+void fft_tables_destroy(FftTables* t);
+
+enum FftLoad { kLoadPlain = 0, kLoadComb = 1 };
+enum FftStore { kStoreCplx = 0, kStoreKeys = 1, kStoreBoth = 2 };
+
+// One canonical FFT of `batch` transforms.  Passes ping-pong between out and scratch so that
+// the last pass lands in `out` (keys for kStoreKeys/kStoreBoth).  in may equal out only for
+// single-pass sizes.  The comb loader reads element j of batch b from
+// x[offs[b] + j * sigma] (pass 0); in is then ignored.
+struct FftLaunch {
+  const FftTables* tab = nullptr;
+  const void* in = nullptr;
+  long long in_bstride = 0;      // elements
+  void* out = nullptr;
+  long long out_bstride = 0;
+  void* scratch = nullptr;       // batch * N elements, stride N
+  void* keys = nullptr;          // u64 (fp64) / u32 (fp32) keys |X|^2, stride keys_bstride
+  long long keys_bstride = 0;
+  int key_batches = 0;
+  int batch = 1;
+  int load = kLoadPlain;
+  int store = kStoreCplx;
+  const void* comb_x = nullptr;
+  const int* comb_offs = nullptr;
+  int comb_sigma = 1;
+};
+hipError_t fft_run(const FftLaunch& L, hipStream_t st);
+
+// ---- permutation + filtering -------------------------------------------------------------
+// bins[l*B + r] = canonical fold of loop l (ai_dev[l]) for r < B, l < loops.
+hipError_t perm_filter_launch(int is_fp32, const void* x, const void* f, void* bins,
+                              const int* ai_dev, int n, int B, int w, int loops, int tiled,
+                              hipStream_t st);
+
+// ---- top-K select over `segs` key segments of N keys (stride seg_stride) ----------------
+struct SelectWork {
+  void* state = nullptr;      // per segment: {u64 prefix, int remaining, int pad}
+  unsigned* hist = nullptr;   // segs * 2048
+  unsigned* done = nullptr;   // segs
+  int* cnt = nullptr;         // segs * nchunks * 2
+  int max_segs = 0;
+  int max_chunks = 0;
+};
+size_t select_work_bytes(int segs, int N);
+void select_work_bind(SelectWork* w, void* base, int segs, int N);
+// out[s*K + j] = ascending indices of the K largest keys of segment s (ties by lower index)
+hipError_t select_topk(int is_fp32, const void* keys, long long seg_stride, int N, int K,
+                       int segs, int* out, const SelectWork& w, hipStream_t st);
+
+// ---- reverse hash (bitmap of approved residues, votes, deterministic hit list) -----------
+struct HashArgs {
+  int n, B, K, W, loops_loc, loops_thresh, bpe;
+  const int* J;          // loops_loc x K
+  const int* J2;         // nJ2 entries (all comb lists, duplicates allowed)
+  int nJ2;
+  const int* a_dev;      // per loop
+  const int* ai_dev;
+  unsigned* bitmap;      // loops_loc x words
+  int bm_words;
+  int* maxfirst;         // loops_loc
+  unsigned* vote;        // n * bpe / 32 words
+  int* hitoff;           // loops_loc x K (local prefix within block)
+  int* blocksum;         // blocks
+  int* blockoff;         // blocks
+  int* hits;             // capacity
+  int capacity;
+  int* count;            // device: total hits (may exceed capacity)
+};
+hipError_t hash_clear_launch(const HashArgs& h, hipStream_t st);
+hipError_t hash_bitmap_launch(const HashArgs& h, hipStream_t st);
+hipError_t hash_vote_launch(const HashArgs& h, hipStream_t st);
+hipError_t hash_hits_launch(const HashArgs& h, hipStream_t st);
+int hash_blocks(int loops_loc, int K);
+
+// ---- estimation ---------------------------------------------------------------------------
+hipError_t estimate_launch(int is_fp32, const int* hits, const int* count, int capacity,
+                           const void* bins_f, int loops, int n, const int* ai_dev, int B,
+                           const double2* filter_win, Node* out, hipStream_t st);
+
+// ---- small utilities ---------------------------------------------------------------------
+hipError_t fill_u32(void* p, uint32_t v, size_t words, hipStream_t st);
+
+}  // namespace sfft

@@ -1,0 +1,630 @@
+// sfft_plan.hip -- host orchestration of one sparse-FFT transform on the GPU (the native plan
+// API of include/sfft.h) and the canonical FFT tables shared by every FFT launch.
+//
+// Per transform, on the plan's stream, with no host round trip until the final copy:
+//   clear votes/bitmaps -> comb FFT (gather fused into pass 0, |Y|^2 keys out of the last
+//   pass) -> top-B_thresh of each comb -> approved-residue bitmaps per location loop ->
+//   permute+filter of every loop -> batched bucket FFT (bins_f + keys of location loops) ->
+//   top-B_thresh per location loop -> votes -> deterministic hit list -> estimation.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "sfft_internal.h"
+
+namespace sfft {
+
+// ---- canonical twiddles: exp(-+2 pi i e/T), octant reduction so cos/sin see [0, pi/4] ----
+void canon_twiddle(uint64_t e, uint64_t T, int inverse, double* cr, double* sr) {
+  double c, s;
+  e %= T;
+  if (T < 8) {
+    const uint64_t e4 = e * (4 / T);
+    static const double cq[4] = {1.0, 0.0, -1.0, 0.0};
+    static const double sq[4] = {0.0, 1.0, 0.0, -1.0};
+    c = cq[e4 & 3];
+    s = sq[e4 & 3];
+  } else {
+    const uint64_t quarter = T / 4;
+    const uint64_t qd = e / quarter;
+    const uint64_t r = e % quarter;
+    const bool swap = 2 * r > quarter;
+    const uint64_t rr = swap ? (quarter - r) : r;
+    const double ang = (2.0 * M_PI * (double)rr) / (double)T;
+    double c0, s0;
+    sincos(ang, &s0, &c0);   // one libm call: sin()/cos() may round differently from it
+    if (swap) {
+      const double t = c0;
+      c0 = s0;
+      s0 = t;
+    }
+    switch (qd) {
+      case 0: c = c0; s = s0; break;
+      case 1: c = -s0; s = c0; break;
+      case 2: c = -c0; s = -s0; break;
+      default: c = s0; s = -c0; break;
+    }
+  }
+  *cr = c;
+  *sr = inverse ? s : -s;
+}
+
+int fft_plan_passes(int log2n, int* pass_logs) {
+  if (log2n <= 0) return 0;
+  const int p = (log2n + kMaxPassLog - 1) / kMaxPassLog;
+  const int base = log2n / p, extra = log2n % p;
+  for (int i = 0; i < p; ++i) pass_logs[i] = base + (i < extra ? 1 : 0);
+  return p;
+}
+
+hipError_t fft_tables_create(int log2n, int inverse, int is_fp32, FftTables* t) {
+  *t = FftTables();
+  t->log2n = log2n;
+  t->inverse = inverse;
+  t->is_fp32 = is_fp32;
+  t->npass = fft_plan_passes(log2n, t->logs);
+  const size_t esz = is_fp32 ? 8 : 16;
+  // layout: per pass [twR (R)] [tdir (Lcur) | thi (Lcur>>s) tlo (1<<s)]
+  size_t cnt[kMaxPasses][4] = {};
+  size_t total = 0;
+  int log2Lprev = 0;
+  for (int p = 0; p < t->npass; ++p) {
+    const int r = t->logs[p];
+    const int log2Lcur = log2Lprev + r;
+    cnt[p][0] = (size_t)1 << r;
+    if (p > 0) {
+      if (log2Lcur <= kMaxPassLog) {
+        cnt[p][1] = (size_t)1 << log2Lcur;
+      } else {
+        t->s[p] = (log2Lcur + 1) / 2;
+        cnt[p][2] = ((size_t)1 << log2Lcur) >> t->s[p];
+        cnt[p][3] = (size_t)1 << t->s[p];
+      }
+    }
+    for (int j = 0; j < 4; ++j) total += cnt[p][j];
+    log2Lprev = log2Lcur;
+  }
+  t->bytes = total * esz;
+  std::vector<unsigned char> host(t->bytes ? t->bytes : 1);
+  size_t off = 0;
+  std::vector<size_t> offs;
+  auto put = [&](uint64_t e, uint64_t T) {
+    double c, s;
+    canon_twiddle(e, T, inverse, &c, &s);
+    if (is_fp32) {
+      float v[2] = {(float)c, (float)s};
+      memcpy(&host[off], v, 8);
+    } else {
+      double v[2] = {c, s};
+      memcpy(&host[off], v, 16);
+    }
+    off += esz;
+  };
+  log2Lprev = 0;
+  size_t start[kMaxPasses][4] = {};
+  for (int p = 0; p < t->npass; ++p) {
+    const int r = t->logs[p];
+    const int log2Lcur = log2Lprev + r;
+    const uint64_t R = 1ull << r, Lcur = 1ull << log2Lcur;
+    start[p][0] = off;
+    for (uint64_t e = 0; e < R; ++e) put(e, R);
+    start[p][1] = off;
+    for (size_t e = 0; e < cnt[p][1]; ++e) put(e, Lcur);
+    start[p][2] = off;
+    for (size_t e = 0; e < cnt[p][2]; ++e) put((uint64_t)e << t->s[p], Lcur);
+    start[p][3] = off;
+    for (size_t e = 0; e < cnt[p][3]; ++e) put(e, Lcur);
+    log2Lprev = log2Lcur;
+  }
+  void* d = nullptr;
+  hipError_t err = hipMalloc(&d, t->bytes ? t->bytes : 16);
+  if (err != hipSuccess) return err;
+  if (t->bytes) {
+    err = hipMemcpy(d, host.data(), t->bytes, hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+      (void)hipFree(d);
+      return err;
+    }
+  }
+  t->storage = d;
+  for (int p = 0; p < t->npass; ++p) {
+    unsigned char* base = (unsigned char*)d;
+    t->twR[p] = base + start[p][0];
+    t->tdir[p] = cnt[p][1] ? base + start[p][1] : nullptr;
+    t->thi[p] = cnt[p][2] ? base + start[p][2] : nullptr;
+    t->tlo[p] = cnt[p][3] ? base + start[p][3] : nullptr;
+  }
+  return hipSuccess;
+}
+
+void fft_tables_destroy(FftTables* t) {
+  if (t->storage) (void)hipFree(t->storage);
+  t->storage = nullptr;
+}
+
+// cache of tables for the standalone device FFT entry point
+struct TabKey {
+  int log2n, inverse, fp32, device;
+};
+static std::mutex g_tab_mu;
+static std::vector<std::pair<TabKey, FftTables*>> g_tabs;
+
+const FftTables* cached_tables(int log2n, int inverse, int fp32, hipError_t* err) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  for (auto& kv : g_tabs)
+    if (kv.first.log2n == log2n && kv.first.inverse == inverse && kv.first.fp32 == fp32 &&
+        kv.first.device == dev)
+      return kv.second;
+  FftTables* t = new FftTables();
+  *err = fft_tables_create(log2n, inverse, fp32, t);
+  if (*err != hipSuccess) {
+    delete t;
+    return nullptr;
+  }
+  g_tabs.push_back({TabKey{log2n, inverse, fp32, dev}, t});
+  return t;
+}
+
+}  // namespace sfft
+
+using namespace sfft;
+
+// ============================================================================================
+// plan
+// ============================================================================================
+struct sfft_plan {
+  sfft_params p;
+  int fp32 = 0;
+  hipStream_t st = nullptr;
+  int log2n = 0, log2B = 0, log2W = 0, m = 0;
+  int bpe = 1;
+  int cap = MAXSIZE_HITS;
+  FftTables tabB, tabW;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  int* d_a = nullptr;
+  int* d_ai = nullptr;
+  int* d_offs = nullptr;
+  void* comb0 = nullptr;
+  void* comb1 = nullptr;
+  void* comb_keys = nullptr;
+  void* bins_t = nullptr;
+  void* bins_scr = nullptr;
+  void* bins_f = nullptr;
+  void* bkeys = nullptr;
+  int* J = nullptr;
+  int* J2 = nullptr;
+  SelectWork selW, selB;
+  unsigned* bitmap = nullptr;
+  int bm_words = 0;
+  int* maxfirst = nullptr;
+  unsigned* vote = nullptr;
+  int* hitoff = nullptr;
+  int* blocksum = nullptr;
+  int* blockoff = nullptr;
+  int* hits = nullptr;
+  int* count = nullptr;
+  double2* win = nullptr;
+  Node* nodes = nullptr;
+  const void* d_ft = nullptr;
+  int has_filter = 0;
+  int last_count = 0;
+  // optional stage timing (hipEvents at the reference's stage boundaries)
+  int profile = 0;
+  hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+static int ilog2_exact(long long v) {
+  if (v <= 0) return -1;
+  int l = 0;
+  while ((1LL << l) < v) ++l;
+  return ((1LL << l) == v) ? l : -1;
+}
+
+extern "C" const char* sfft_status_string(sfft_status s) {
+  switch (s) {
+    case SFFT_OK: return "ok";
+    case SFFT_ERR_INVALID: return "invalid parameters";
+    case SFFT_ERR_ALLOC: return "device allocation failed";
+    case SFFT_ERR_HIP: return "HIP runtime error";
+    case SFFT_ERR_HIT_OVERFLOW: return "located indices exceed hit capacity";
+    case SFFT_ERR_NO_FILTER: return "plan has no filter";
+    case SFFT_ERR_NO_DEVICE: return "no HIP device";
+  }
+  return "unknown";
+}
+
+extern "C" sfft_status sfft_params_validate(const sfft_params* p) {
+  if (!p) return SFFT_ERR_INVALID;
+  const int ln = ilog2_exact(p->n), lB = ilog2_exact(p->B), lW = ilog2_exact(p->W_RLFC);
+  if (ln < 5 || ln > 30 || lB < 1 || lW < 5) return SFFT_ERR_INVALID;
+  if (p->B > p->n || p->W_RLFC > p->n) return SFFT_ERR_INVALID;
+  if (!(p->B_thresh >= 1 && p->B_thresh < p->B)) return SFFT_ERR_INVALID;       // experiment.c:83
+  if (!(p->B_thresh < p->W_RLFC)) return SFFT_ERR_INVALID;                       // experiment.c:82
+  if (!(p->loops_thresh >= 1 && p->loops_thresh <= p->loops_loc)) return SFFT_ERR_INVALID;
+  if (p->loops_loc < 1 || p->loops_loc > 32 || p->loops < p->loops_loc || p->loops > 32)
+    return SFFT_ERR_INVALID;
+  if (p->RLFC_loops < 1 || p->RLFC_loops > 64) return SFFT_ERR_INVALID;
+  if (p->w < 1 || p->w > p->n) return SFFT_ERR_INVALID;                          // cuExperiment.cu:283
+  if (p->fold_mode == SFFT_FOLD_REFERENCE && p->n < (1 << 27) && p->w < p->B)
+    return SFFT_ERR_INVALID;   // the tiled fold's tail taps would index past w
+  if (p->fold_mode != SFFT_FOLD_REFERENCE && p->fold_mode != SFFT_FOLD_ALL_TAPS)
+    return SFFT_ERR_INVALID;
+  if (p->hit_capacity < 0) return SFFT_ERR_INVALID;
+  if (p->W_RLFC < p->n / p->B) return SFFT_ERR_INVALID;   // the comb block must span a window
+  return SFFT_OK;
+}
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision prec,
+                                        void* stream, sfft_plan** out) {
+  if (!out) return SFFT_ERR_INVALID;
+  *out = nullptr;
+  sfft_status v = sfft_params_validate(prm);
+  if (v != SFFT_OK) return v;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SFFT_ERR_NO_DEVICE;
+  sfft_plan* P = new sfft_plan();
+  P->p = *prm;
+  P->fp32 = (prec == SFFT_FP32);
+  P->st = (hipStream_t)stream;
+  P->log2n = ilog2_exact(prm->n);
+  P->log2B = ilog2_exact(prm->B);
+  P->log2W = ilog2_exact(prm->W_RLFC);
+  P->m = prm->n / prm->B;
+  P->cap = prm->hit_capacity > 0 ? prm->hit_capacity : MAXSIZE_HITS;
+  P->bpe = 1;
+  while (P->bpe < prm->loops_loc) P->bpe <<= 1;
+  const int L = prm->loops, Ll = prm->loops_loc, Rl = prm->RLFC_loops, K = prm->B_thresh;
+  const size_t W = prm->W_RLFC, B = prm->B;
+  const size_t e = P->fp32 ? 8 : 16, ke = P->fp32 ? 4 : 8;
+  P->bm_words = (int)((W + 31) / 32);
+  const int nhb = hash_blocks(Ll, K);
+  // carve one workspace
+  struct Item {
+    void** dst;
+    size_t bytes;
+  };
+  std::vector<Item> items;
+  items.push_back({(void**)&P->d_a, (size_t)L * 4});
+  items.push_back({(void**)&P->d_ai, (size_t)L * 4});
+  items.push_back({(void**)&P->d_offs, (size_t)Rl * 4});
+  items.push_back({&P->comb0, (size_t)Rl * W * e});
+  items.push_back({&P->comb1, (size_t)Rl * W * e});
+  items.push_back({&P->comb_keys, (size_t)Rl * W * ke});
+  items.push_back({&P->bins_t, (size_t)L * B * e});
+  items.push_back({&P->bins_scr, (size_t)L * B * e});
+  items.push_back({&P->bins_f, (size_t)L * B * e});
+  items.push_back({&P->bkeys, (size_t)Ll * B * ke});
+  items.push_back({(void**)&P->J, (size_t)Ll * K * 4});
+  items.push_back({(void**)&P->J2, (size_t)Rl * K * 4});
+  void* selW_base = nullptr;
+  void* selB_base = nullptr;
+  items.push_back({&selW_base, select_work_bytes(Rl, (int)W)});
+  items.push_back({&selB_base, select_work_bytes(Ll, (int)B)});
+  items.push_back({(void**)&P->bitmap, (size_t)Ll * P->bm_words * 4});
+  items.push_back({(void**)&P->maxfirst, (size_t)Ll * 4});
+  items.push_back({(void**)&P->vote, ((size_t)prm->n * P->bpe + 31) / 32 * 4});
+  items.push_back({(void**)&P->hitoff, (size_t)Ll * K * 4});
+  items.push_back({(void**)&P->blocksum, (size_t)nhb * 4});
+  items.push_back({(void**)&P->blockoff, (size_t)nhb * 4});
+  items.push_back({(void**)&P->hits, (size_t)P->cap * 4});
+  items.push_back({(void**)&P->count, 16});
+  items.push_back({(void**)&P->win, (size_t)P->m * 16});
+  items.push_back({(void**)&P->nodes, (size_t)P->cap * sizeof(Node)});
+  size_t total = 0;
+  for (auto& it : items) total = align_up(total, 256) + align_up(it.bytes, 256);
+  if (hipMalloc(&P->ws, total) != hipSuccess) {
+    delete P;
+    return SFFT_ERR_ALLOC;
+  }
+  P->ws_bytes = total;
+  size_t off = 0;
+  for (auto& it : items) {
+    off = align_up(off, 256);
+    *it.dst = (char*)P->ws + off;
+    off += align_up(it.bytes, 256);
+  }
+  select_work_bind(&P->selW, selW_base, Rl, (int)W);
+  select_work_bind(&P->selB, selB_base, Ll, (int)B);
+  // select state must start at zero (hist/done are reset by the kernels themselves)
+  if (hipMemset(P->ws, 0, total) != hipSuccess ||
+      fft_tables_create(P->log2B, 0, P->fp32, &P->tabB) != hipSuccess ||
+      fft_tables_create(P->log2W, 0, P->fp32, &P->tabW) != hipSuccess) {
+    fft_tables_destroy(&P->tabB);
+    fft_tables_destroy(&P->tabW);
+    (void)hipFree(P->ws);
+    delete P;
+    return SFFT_ERR_ALLOC;
+  }
+  *out = P;
+  return SFFT_OK;
+}
+
+extern "C" void sfft_plan_destroy(sfft_plan* P) {
+  if (!P) return;
+  for (auto& e : P->ev)
+    if (e) (void)hipEventDestroy(e);
+  fft_tables_destroy(&P->tabB);
+  fft_tables_destroy(&P->tabW);
+  if (P->ws) (void)hipFree(P->ws);
+  delete P;
+}
+
+extern "C" size_t sfft_plan_workspace_bytes(const sfft_plan* P) { return P ? P->ws_bytes : 0; }
+
+extern "C" sfft_status sfft_plan_set_filter(sfft_plan* P, const void* d_filter_time,
+                                            const complex_t* h_filter_win) {
+  if (!P || !d_filter_time || !h_filter_win) return SFFT_ERR_INVALID;
+  if (hipMemcpy(P->win, h_filter_win, (size_t)P->m * 16, hipMemcpyHostToDevice) != hipSuccess)
+    return SFFT_ERR_HIP;
+  P->d_ft = d_filter_time;
+  P->has_filter = 1;
+  return SFFT_OK;
+}
+
+static sfft_status draws_check(const sfft_plan* P, const sfft_draws* d) {
+  if (!d || !d->a || !d->ai || !d->offsets) return SFFT_ERR_INVALID;
+  const long long n = P->p.n;
+  const int sigma = P->p.n / P->p.W_RLFC;
+  for (int l = 0; l < P->p.loops; ++l) {
+    if (d->a[l] <= 0 || d->a[l] >= n || (d->a[l] & 1) == 0) return SFFT_ERR_INVALID;
+    if (d->ai[l] <= 0 || d->ai[l] >= n) return SFFT_ERR_INVALID;
+    if ((((long long)d->a[l] * d->ai[l]) & (n - 1)) != 1) return SFFT_ERR_INVALID;
+  }
+  for (int r = 0; r < P->p.RLFC_loops; ++r)
+    if (d->offsets[r] < 0 || d->offsets[r] >= sigma) return SFFT_ERR_INVALID;
+  return SFFT_OK;
+}
+
+#define HIPCK(x)                            \
+  do {                                      \
+    if ((x) != hipSuccess) return SFFT_ERR_HIP; \
+  } while (0)
+
+extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const sfft_draws* dr,
+                                          Node* d_nodes, int* d_count) {
+  if (!P || !d_x) return SFFT_ERR_INVALID;
+  if (!P->has_filter) return SFFT_ERR_NO_FILTER;
+  sfft_status v = draws_check(P, dr);
+  if (v != SFFT_OK) return v;
+  const sfft_params& p = P->p;
+  hipStream_t st = P->st;
+  const int L = p.loops, Ll = p.loops_loc, Rl = p.RLFC_loops, K = p.B_thresh;
+  const long long W = p.W_RLFC, B = p.B;
+  // draws (pageable source: the copy is staged before hipMemcpyAsync returns)
+  HIPCK(hipMemcpyAsync(P->d_a, dr->a, (size_t)L * 4, hipMemcpyHostToDevice, st));
+  HIPCK(hipMemcpyAsync(P->d_ai, dr->ai, (size_t)L * 4, hipMemcpyHostToDevice, st));
+  HIPCK(hipMemcpyAsync(P->d_offs, dr->offsets, (size_t)Rl * 4, hipMemcpyHostToDevice, st));
+
+  HashArgs h;
+  h.n = p.n;
+  h.B = p.B;
+  h.K = K;
+  h.W = p.W_RLFC;
+  h.loops_loc = Ll;
+  h.loops_thresh = p.loops_thresh;
+  h.bpe = P->bpe;
+  h.J = P->J;
+  h.J2 = P->J2;
+  h.nJ2 = Rl * K;
+  h.a_dev = P->d_a;
+  h.ai_dev = P->d_ai;
+  h.bitmap = P->bitmap;
+  h.bm_words = P->bm_words;
+  h.maxfirst = P->maxfirst;
+  h.vote = P->vote;
+  h.hitoff = P->hitoff;
+  h.blocksum = P->blocksum;
+  h.blockoff = P->blockoff;
+  h.hits = P->hits;
+  h.capacity = P->cap;
+  h.count = d_count ? d_count : P->count;
+  auto mark = [&](int i) -> hipError_t {
+    return P->profile ? hipEventRecord(P->ev[i], st) : hipSuccess;
+  };
+  HIPCK(mark(0));
+  HIPCK(hash_clear_launch(h, st));
+  HIPCK(mark(1));
+
+  // RLFC: comb gather + W-point FFT, keys of |Y|^2 out of the last pass
+  FftLaunch c;
+  c.tab = &P->tabW;
+  c.out = P->comb0;
+  c.out_bstride = W;
+  c.scratch = P->comb1;
+  c.keys = P->comb_keys;
+  c.keys_bstride = W;
+  c.key_batches = Rl;
+  c.batch = Rl;
+  c.load = kLoadComb;
+  c.store = kStoreKeys;
+  c.comb_x = d_x;
+  c.comb_offs = P->d_offs;
+  c.comb_sigma = (int)(p.n / p.W_RLFC);
+  HIPCK(fft_run(c, st));
+  HIPCK(select_topk(P->fp32, P->comb_keys, W, (int)W, K, Rl, P->J2, P->selW, st));
+  HIPCK(hash_bitmap_launch(h, st));
+  HIPCK(mark(2));
+
+  // permutation + filtering of every loop, then the batched bucket FFT
+  const int tiled = (p.fold_mode == SFFT_FOLD_REFERENCE) && (p.n < (1 << 27));
+  HIPCK(perm_filter_launch(P->fp32, d_x, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, L, tiled,
+                           st));
+  HIPCK(mark(3));
+  FftLaunch f;
+  f.tab = &P->tabB;
+  f.in = P->bins_t;
+  f.in_bstride = B;
+  f.out = P->bins_f;
+  f.out_bstride = B;
+  f.scratch = P->bins_scr;
+  f.keys = P->bkeys;
+  f.keys_bstride = B;
+  f.key_batches = Ll;
+  f.batch = L;
+  f.store = kStoreBoth;
+  if (P->log2B == 0) return SFFT_ERR_INVALID;
+  HIPCK(fft_run(f, st));
+  HIPCK(select_topk(P->fp32, P->bkeys, B, (int)B, K, Ll, P->J, P->selB, st));
+  HIPCK(mark(4));
+
+  // reverse hash: votes, then the hit list in (loop, bucket, walk) order
+  HIPCK(hash_vote_launch(h, st));
+  HIPCK(hash_hits_launch(h, st));
+  HIPCK(mark(5));
+
+  // estimation
+  HIPCK(estimate_launch(P->fp32, P->hits, h.count, P->cap, P->bins_f, L, p.n, P->d_ai, p.B,
+                        P->win, d_nodes ? d_nodes : P->nodes, st));
+  HIPCK(mark(6));
+  return SFFT_OK;
+}
+
+extern "C" sfft_status sfft_plan_set_profiling(sfft_plan* P, int on) {
+  if (!P) return SFFT_ERR_INVALID;
+  if (on)
+    for (auto& e : P->ev)
+      if (!e && hipEventCreate(&e) != hipSuccess) return SFFT_ERR_HIP;
+  P->profile = on ? 1 : 0;
+  return SFFT_OK;
+}
+
+// Seconds per stage of the last profiled transform, in the reference's table order
+// (outerLoop.cu:174-187): votetable, restr-loc-index, perm+filter, reverse-hash, estimation,
+// FFT+Cutoff, total.  Call after the transform completed.
+extern "C" sfft_status sfft_plan_stage_times(sfft_plan* P, double* out7) {
+  if (!P || !out7 || !P->profile) return SFFT_ERR_INVALID;
+  float t[6];
+  for (int i = 0; i < 6; ++i)
+    if (hipEventElapsedTime(&t[i], P->ev[i], P->ev[i + 1]) != hipSuccess) return SFFT_ERR_HIP;
+  out7[0] = t[0] * 1e-3;  // votetable: vote/bitmap clear
+  out7[1] = t[1] * 1e-3;  // restr-loc-index: comb FFT + cutoff + bitmaps
+  out7[2] = t[2] * 1e-3;  // perm+filter
+  out7[5] = t[3] * 1e-3;  // FFT+Cutoff: bucket FFT + location cutoffs
+  out7[3] = t[4] * 1e-3;  // reverse-hash
+  out7[4] = t[5] * 1e-3;  // estimation
+  float tot = 0;
+  if (hipEventElapsedTime(&tot, P->ev[0], P->ev[6]) != hipSuccess) return SFFT_ERR_HIP;
+  out7[6] = tot * 1e-3;
+  return SFFT_OK;
+}
+
+extern "C" sfft_status sfft_execute(sfft_plan* P, const void* d_x, const sfft_draws* dr,
+                                    Node* h_nodes, int* h_count) {
+  if (!P) return SFFT_ERR_INVALID;
+  sfft_status s = sfft_execute_async(P, d_x, dr, P->nodes, P->count);
+  if (s != SFFT_OK) return s;
+  int cnt = 0;
+  HIPCK(hipMemcpyAsync(&cnt, P->count, 4, hipMemcpyDeviceToHost, P->st));
+  HIPCK(hipStreamSynchronize(P->st));
+  P->last_count = cnt;
+  if (h_count) *h_count = cnt;
+  const int ncopy = cnt < P->cap ? cnt : P->cap;
+  if (h_nodes && ncopy > 0) {
+    HIPCK(hipMemcpyAsync(h_nodes, P->nodes, (size_t)ncopy * sizeof(Node), hipMemcpyDeviceToHost,
+                         P->st));
+    HIPCK(hipStreamSynchronize(P->st));
+  }
+  return cnt > P->cap ? SFFT_ERR_HIT_OVERFLOW : SFFT_OK;
+}
+
+extern "C" const void* sfft_plan_debug_buffer(const sfft_plan* P, int which) {
+  if (!P) return nullptr;
+  switch (which) {
+    case 0: return P->bins_f;
+    case 1: return P->J;
+    case 2: return P->J2;
+    case 3: return P->hits;
+    case 4: return P->bins_t;
+    case 5: return P->nodes;
+    case 6: return P->count;
+  }
+  return nullptr;
+}
+
+// ============================================================================================
+// standalone canonical FFT
+// ============================================================================================
+extern "C" sfft_status sfft_fft_device(const void* d_in, void* d_out, int log2n, int count,
+                                       int inverse, sfft_precision prec, void* stream) {
+  if (log2n < 0 || log2n > 30 || count < 1 || !d_in || !d_out) return SFFT_ERR_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t N = (size_t)1 << log2n;
+  const size_t esz = prec == SFFT_FP32 ? 8 : 16;
+  if (log2n == 0) {
+    if (d_in != d_out)
+      HIPCK(hipMemcpyAsync(d_out, d_in, (size_t)count * esz, hipMemcpyDeviceToDevice, st));
+    return SFFT_OK;
+  }
+  hipError_t err = hipSuccess;
+  const FftTables* T = cached_tables(log2n, inverse ? 1 : 0, prec == SFFT_FP32, &err);
+  if (!T) return SFFT_ERR_ALLOC;
+  void* scratch = nullptr;
+  const void* in = d_in;
+  void* tmp_in = nullptr;
+  if (T->npass > 1) {
+    if (hipMallocAsync(&scratch, N * count * esz, st) != hipSuccess) return SFFT_ERR_ALLOC;
+    if (d_in == d_out) {
+      // multi-pass needs in != out: stage the input
+      if (hipMallocAsync(&tmp_in, N * count * esz, st) != hipSuccess) return SFFT_ERR_ALLOC;
+      HIPCK(hipMemcpyAsync(tmp_in, d_in, N * count * esz, hipMemcpyDeviceToDevice, st));
+      in = tmp_in;
+    }
+  }
+  FftLaunch L;
+  L.tab = T;
+  L.in = in;
+  L.in_bstride = (long long)N;
+  L.out = d_out;
+  L.out_bstride = (long long)N;
+  L.scratch = scratch;
+  L.batch = count;
+  L.load = kLoadPlain;
+  L.store = kStoreCplx;
+  err = fft_run(L, st);
+  if (scratch) (void)hipFreeAsync(scratch, st);
+  if (tmp_in) (void)hipFreeAsync(tmp_in, st);
+  return err == hipSuccess ? SFFT_OK : SFFT_ERR_HIP;
+}
+
+extern "C" void sfft_twiddle(uint64_t e, uint64_t T, int inverse, double* re, double* im) {
+  canon_twiddle(e, T, inverse, re, im);
+}
+
+extern "C" int sfft_fft_plan_passes(int log2n, int* pass_logs) {
+  return fft_plan_passes(log2n, pass_logs);
+}
+
+extern "C" const char* sfft_version(void) { return "sfft-mi355x 0.1.0 (gfx950)"; }
+
+extern "C" int sfft_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+namespace sfft {
+hipError_t fft_plain(const void* in, void* out, void* scratch, int log2n, int count, int inverse,
+                     int is_fp32, hipStream_t st) {
+  hipError_t err = hipSuccess;
+  const FftTables* T = cached_tables(log2n, inverse ? 1 : 0, is_fp32, &err);
+  if (!T) return err == hipSuccess ? hipErrorOutOfMemory : err;
+  const long long N = 1LL << log2n;
+  FftLaunch L;
+  L.tab = T;
+  L.in = in;
+  L.in_bstride = N;
+  L.out = out;
+  L.out_bstride = N;
+  L.scratch = scratch;
+  L.batch = count;
+  return fft_run(L, st);
+}
+}  // namespace sfft

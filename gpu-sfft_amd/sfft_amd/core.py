@@ -1,0 +1,333 @@
+"""Host-side mirror of the reference interface over the C ABI.
+
+Device memory is held in torch tensors (complex128 = complex_t, complex64 = float2); only
+raw pointers cross into libsfft.  Parameter derivation follows experiment.c:54-84 and
+cuExperiment.cu:279-284; the transform follows outerLoop.cu:65-190 with explicit draws.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import lib
+
+NODE_DTYPE = np.dtype([("key", np.uint32), ("pad", np.uint32), ("re", np.float64),
+                       ("im", np.float64)])
+assert NODE_DTYPE.itemsize == C.sizeof(L.Node) == 24
+
+FOLD_REFERENCE = L.SFFT_FOLD_REFERENCE
+FOLD_ALL_TAPS = L.SFFT_FOLD_ALL_TAPS
+
+
+class SfftError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = lib.sfft_status_string(status).decode()
+        super().__init__(f"{what}: {msg} (status {status})" if what else msg)
+
+
+def _check(status, what):
+    if status != L.SFFT_OK:
+        raise SfftError(status, what)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_handle(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---- parameters (experiment.c:54-84) -------------------------------------------------------
+def floor_to_pow2(x):
+    ans = 1
+    while ans <= x:
+        ans <<= 1
+    return ans // 2
+
+
+def derive_params(q, k, Bcst, loops_loc, loops_thresh, loops_est, RLFC_cst, RLFC_loops,
+                  tolerance=1e-8):
+    """The reference's parameter derivation (experiment.c:54-84; w: cuExperiment.cu:279-284)."""
+    n = floor_to_pow2(int(math.pow(2, q)))
+    RLFC_cst = float(int(RLFC_cst))                      # atoi (experiment.c:62)
+    BB = float(int(Bcst * math.sqrt(float(n) * k / math.log2(n))))   # (unsigned) cast
+    lobefrac = 0.5 / BB
+    b_f = int(1.2 * 1.1 * (float(n) / BB))
+    B = floor_to_pow2(BB)
+    B_thresh = 2 * k
+    W = floor_to_pow2(RLFC_cst * n / B)
+    t0 = C.c_double()
+    w = lib.sfft_filter_taps(lobefrac, tolerance, C.byref(t0))
+    ok = (b_f <= n) and (B_thresh < W) and (B_thresh < B) and (loops_thresh <= loops_loc) \
+        and (w <= n)
+    return dict(q=q, n=n, k=k, Bcst=Bcst, loops_loc=loops_loc, loops_thresh=loops_thresh,
+                loops_est=loops_est, loops=loops_loc + loops_est, RLFC_cst=RLFC_cst,
+                RLFC_loops=RLFC_loops, tolerance=tolerance, BB=BB, lobefrac=lobefrac, b_f=b_f,
+                B=B, B_thresh=B_thresh, W_RLFC=W, w=w, t0=t0.value, assert_ok=ok)
+
+
+def draw_permutations(n, loops, RLFC_loops, W, seed):
+    """Seeded draws: a odd uniform on [1, n), ai = a^-1 mod n, comb offsets on [0, n/W)."""
+    rng = np.random.default_rng(seed)
+    a = (rng.integers(0, n // 2, size=loops, dtype=np.int64) * 2 + 1).astype(np.int64)
+    ai = np.array([pow(int(v), -1, n) for v in a], dtype=np.int64)
+    offs = rng.integers(0, n // W, size=RLFC_loops, dtype=np.int64)
+    return a.astype(np.int32), ai.astype(np.int32), offs.astype(np.int32)
+
+
+def validate(params, fold_mode=FOLD_REFERENCE, hit_capacity=0):
+    p = _cparams(params, fold_mode, hit_capacity)
+    return lib.sfft_params_validate(C.byref(p))
+
+
+def _cparams(params, fold_mode, hit_capacity):
+    return L.Params(n=params["n"], B=params["B"], B_thresh=params["B_thresh"],
+                    W_RLFC=params["W_RLFC"], RLFC_loops=params["RLFC_loops"],
+                    loops_thresh=params["loops_thresh"], loops_loc=params["loops_loc"],
+                    loops=params["loops"], w=params["w"], fold_mode=fold_mode,
+                    hit_capacity=hit_capacity)
+
+
+# ---- plan ------------------------------------------------------------------------------------
+class Plan:
+    """One transform configuration on the current device (outer_loop with explicit draws)."""
+
+    def __init__(self, params, precision="fp64", fold_mode=FOLD_REFERENCE, hit_capacity=0,
+                 stream=None):
+        self.params = dict(params)
+        self.precision = precision
+        self.fp32 = precision == "fp32"
+        self.stream = stream
+        self._cp = _cparams(params, fold_mode, hit_capacity)
+        self.capacity = hit_capacity if hit_capacity > 0 else 524288
+        h = C.c_void_p()
+        _check(lib.sfft_plan_create(C.byref(self._cp), L.SFFT_FP32 if self.fp32 else L.SFFT_FP64,
+                                    _stream_handle(stream), C.byref(h)), "sfft_plan_create")
+        self._h = h
+        self._ft = None
+        self._nodes = np.zeros(self.capacity, dtype=NODE_DTYPE)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.sfft_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def workspace_bytes(self):
+        return lib.sfft_plan_workspace_bytes(self._h)
+
+    def set_filter(self, filter_time, filter_win):
+        """filter_time: device tensor (complex128 for fp64 plans, complex64 for fp32) of w taps.
+        filter_win: numpy complex128[m], win[j] = filter_freq[(j - m/2) mod n]."""
+        torch = _torch()
+        want = torch.complex64 if self.fp32 else torch.complex128
+        if filter_time.dtype != want or not filter_time.is_cuda:
+            raise TypeError(f"filter_time must be a {want} device tensor")
+        if filter_time.numel() != self.params["w"]:
+            raise ValueError("filter_time has the wrong number of taps")
+        win = np.ascontiguousarray(filter_win, dtype=np.complex128)
+        if win.size != self.params["n"] // self.params["B"]:
+            raise ValueError("filter window must hold n/B entries")
+        self._ft = filter_time
+        self._win = win
+        _check(lib.sfft_plan_set_filter(self._h, _ptr(filter_time), _np_ptr(win)),
+               "sfft_plan_set_filter")
+
+    def set_profiling(self, on=True):
+        _check(lib.sfft_plan_set_profiling(self._h, int(on)), "sfft_plan_set_profiling")
+
+    def stage_times(self):
+        out = np.zeros(7, dtype=np.float64)
+        _check(lib.sfft_plan_stage_times(self._h, _np_ptr(out)), "sfft_plan_stage_times")
+        return dict(zip(["votetable", "restr-loc-index", "perm+filter", "reverse-hash",
+                         "estimation", "FFT+Cutoff", "total"], out.tolist()))
+
+    def _draws(self, a, ai, offsets):
+        self._a = np.ascontiguousarray(a, dtype=np.int32)
+        self._ai = np.ascontiguousarray(ai, dtype=np.int32)
+        self._off = np.ascontiguousarray(offsets, dtype=np.int32)
+        ip = C.POINTER(C.c_int)
+        return L.Draws(self._a.ctypes.data_as(ip), self._ai.ctypes.data_as(ip),
+                       self._off.ctypes.data_as(ip))
+
+    def _check_x(self, x):
+        torch = _torch()
+        want = torch.complex64 if self.fp32 else torch.complex128
+        if x.dtype != want or not x.is_cuda or x.numel() != self.params["n"] or \
+                not x.is_contiguous():
+            raise TypeError(f"x must be a contiguous {want} device tensor of n entries")
+
+    def execute(self, x, a, ai, offsets):
+        """Runs one transform and returns (nodes[I_F] numpy structured array, I_F)."""
+        self._check_x(x)
+        dr = self._draws(a, ai, offsets)
+        cnt = C.c_int()
+        st = lib.sfft_execute(self._h, _ptr(x), C.byref(dr), _np_ptr(self._nodes),
+                              C.byref(cnt))
+        _check(st, "sfft_execute")
+        return self._nodes[:cnt.value].copy(), cnt.value
+
+    def execute_async(self, x, a, ai, offsets, d_nodes=None, d_count=None):
+        """Enqueues one transform on the plan stream; results stay on the device."""
+        self._check_x(x)
+        dr = self._draws(a, ai, offsets)
+        st = lib.sfft_execute_async(self._h, _ptr(x), C.byref(dr),
+                                    _ptr(d_nodes) if d_nodes is not None else None,
+                                    _ptr(d_count) if d_count is not None else None)
+        _check(st, "sfft_execute_async")
+
+    def debug_array(self, which, count, dtype):
+        """Copies `count` items of a plan-owned device buffer to a numpy array
+        (0 bins_f, 1 J, 2 J2 lists, 3 hits, 4 bins_t, 5 nodes, 6 count)."""
+        torch = _torch()
+        ptr = lib.sfft_plan_debug_buffer(self._h, which)
+        out = np.empty(count, dtype=dtype)
+        torch.cuda.synchronize()
+        err = _hip().hipMemcpy(_np_ptr(out), C.c_void_p(ptr), C.c_size_t(out.nbytes), 2)
+        if err != 0:
+            raise RuntimeError(f"hipMemcpy failed ({err})")
+        return out
+
+
+_hip_lib = None
+
+
+def _hip():
+    global _hip_lib
+    if _hip_lib is None:
+        _hip_lib = C.CDLL("libamdhip64.so")
+        _hip_lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip_lib.hipMemcpy.restype = C.c_int
+    return _hip_lib
+
+
+# ---- filter design (make_multiple_t) -------------------------------------------------------
+def filter_taps(lobefrac, tolerance=1e-8):
+    return lib.sfft_filter_taps(lobefrac, tolerance, None)
+
+
+def make_filter(lobefrac, tolerance, n, b_f, w=None):
+    """FP64 filter of make_multiple_t: (filter_time device complex128[w], filter_freq numpy
+    complex128[n])."""
+    torch = _torch()
+    if w is None:
+        w = filter_taps(lobefrac, tolerance)
+    ft = torch.empty(w, dtype=torch.complex128, device="cuda")
+    ff = np.empty(n, dtype=np.complex128)
+    torch.cuda.synchronize()
+    _check(lib.sfft_make_filter(lobefrac, tolerance, n, b_f, w, _ptr(ft), _np_ptr(ff),
+                                _stream_handle()), "sfft_make_filter")
+    return ft, ff
+
+
+def filter_window(filter_freq, n, B):
+    m = n // B
+    idx = (np.arange(m, dtype=np.int64) - m // 2) % n
+    return np.ascontiguousarray(filter_freq[idx])
+
+
+def to_complex64(x):
+    torch = _torch()
+    out = torch.empty(x.numel(), dtype=torch.complex64, device=x.device)
+    _check(lib.sfft_convert_c128_to_c64(_ptr(x), _ptr(out), x.numel(), _stream_handle()),
+           "convert")
+    return out
+
+
+# ---- signals, truth, metric ------------------------------------------------------------------
+def signal_spectrum(n, k, seed):
+    locs = np.empty(k, dtype=np.int32)
+    vals = np.empty(k, dtype=np.complex128)
+    _check(lib.sfft_signal_spectrum(n, k, seed, _np_ptr(locs), _np_ptr(vals)), "spectrum")
+    return locs, vals
+
+
+def synthesize(n, k, locs, vals, snr_db=0.0, noise_seed=0, fp32=False):
+    torch = _torch()
+    x = torch.empty(n, dtype=torch.complex64 if fp32 else torch.complex128, device="cuda")
+    locs = np.ascontiguousarray(locs, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.complex128)
+    torch.cuda.synchronize()
+    _check(lib.sfft_signal_synthesize(n, k, _np_ptr(locs), _np_ptr(vals), float(snr_db),
+                                      noise_seed, int(fp32), _ptr(x), _stream_handle()),
+           "synthesize")
+    return x
+
+
+def truth_spectrum(x):
+    torch = _torch()
+    n = x.numel()
+    X = torch.empty(n, dtype=torch.complex128, device=x.device)
+    torch.cuda.synchronize()
+    _check(lib.sfft_truth_spectrum(_ptr(x), n, int(x.dtype == torch.complex64), _ptr(X),
+                                   _stream_handle()), "truth_spectrum")
+    return X
+
+
+def metric(nodes, k, locs, X):
+    """run_experiment's metric against the truth spectrum X (device complex128[n])."""
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    locs = np.ascontiguousarray(locs, dtype=np.int32)
+    me, mr = C.c_int(), C.c_int()
+    et, es, ma = C.c_double(), C.c_double(), C.c_double()
+    _check(lib.sfft_metric(_np_ptr(nodes), nodes.size, k, _np_ptr(locs), _ptr(X), X.numel(),
+                           C.byref(me), C.byref(mr), C.byref(et), C.byref(es), C.byref(ma),
+                           _stream_handle()), "metric")
+    return dict(missed_est=me.value, missed_res=mr.value, mse_k=et.value, mse=et.value / k,
+                support_err=es.value, max_abs=ma.value)
+
+
+# ---- canonical FFT -----------------------------------------------------------------------------
+def fft(x, inverse=False, out=None):
+    """Canonical power-of-two FFT of the last axis of a contiguous device tensor."""
+    torch = _torch()
+    n = x.shape[-1]
+    l2 = n.bit_length() - 1
+    if 1 << l2 != n:
+        raise ValueError("length must be a power of two")
+    count = x.numel() // n
+    if out is None:
+        out = torch.empty_like(x)
+    prec = L.SFFT_FP32 if x.dtype == torch.complex64 else L.SFFT_FP64
+    _check(lib.sfft_fft_device(_ptr(x), _ptr(out), l2, count, int(inverse), prec,
+                               _stream_handle()), "sfft_fft_device")
+    return out
+
+
+def dft_any(x, inverse=False):
+    """In-place arbitrary-length FP64 DFT (Bluestein) of a device complex128 vector."""
+    _check(lib.sfft_dft_any_device(_ptr(x), x.numel(), int(inverse), _stream_handle()),
+           "sfft_dft_any_device")
+    return x
+
+
+def twiddle(e, T, inverse=False):
+    re, im = C.c_double(), C.c_double()
+    lib.sfft_twiddle(e, T, int(inverse), C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def fft_plan_passes(log2n):
+    out = (C.c_int * 8)()
+    n = lib.sfft_fft_plan_passes(log2n, out)
+    return [out[i] for i in range(n)]

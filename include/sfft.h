@@ -1,0 +1,206 @@
+/*
+ * sfft.h -- C ABI of libsfft, an MI355X-native sparse FFT (gfx950, hand-written HIP kernels).
+ *
+ * Two layers live behind this header.
+ *
+ * 1. The reference's own entry points, with the reference's names, argument meaning and POD
+ *    layouts, so a caller of pcdslab/gpu-sfft links against libsfft.so unchanged.  Each one
+ *    cites the declaration it replaces.  Device pointers are HIP device memory.
+ *
+ * 2. A native plan API (sfft_plan_*): explicit stream, plan-owned workspace, explicit
+ *    permutation/offset draws, status codes instead of exit()/assert().
+ *
+ * No function here takes a torch type; every pointer is a plain host or device address.
+ */
+#ifndef SFFT_H
+#define SFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- POD types, byte-identical to the reference (utils.h:32-51) -------------------------- */
+typedef struct double2C {
+  double x;
+  double y;
+} complex_t;                                   /* utils.h:32-35, 16 B, = hipDoubleComplex */
+
+typedef int bool_t;                            /* utils.h:37 */
+#ifndef MAXSIZE_HITS
+#define MAXSIZE_HITS 524288                    /* utils.h:40 */
+#endif
+
+typedef struct nodes {
+  unsigned int key;
+  complex_t value;
+} Node;                                        /* utils.h:42-45, 24 B (4 B padding) */
+
+typedef struct pairs {
+  int first;
+  int second;
+} Pair;                                        /* utils.h:48-51 */
+
+/* globals of outerLoop.cu:49-51 (declared in outerLoop.h:29-31) */
+extern bool_t ALGORITHM1;
+extern bool_t VERBOSE;
+extern bool_t TIMING;
+
+/* ---- reference entry points ------------------------------------------------------------- */
+
+/* outerLoop.h:35-38.  d_x: device complex_t[n].  d_filter_time: device complex_t[w].
+ * d_filter_freq: complex_t[n], host-readable or device (only the m = n/B entries around 0
+ * are read).  Returns a malloc'd host Node[*I_F] (caller frees), NULL on failure.
+ * Permutations and RLFC offsets are drawn with libc random()/drand48() in the reference's
+ * order (outerLoop.cu:128-134, innerLoop.cu:122-123). */
+Node *outer_loop(complex_t *d_x, int n, complex_t *d_filter_time, complex_t *d_filter_freq,
+                 int d_filter_size, int B, int B_thresh, int W_RLFC, int RLFC_loops,
+                 int loop_threshold, int loops_loc, int loops, int *I_F, float t_H_to_D);
+
+/* innerLoop.h:32-33.  Writes B_thresh ascending residues to device int d_J2[B_thresh];
+ * adds the elapsed seconds to *RLFC_Time. */
+int restr_loc_freq_coeff(complex_t *d_x, int n, int B_thresh, int W_RLFC, int *d_J2,
+                         double *RLFC_Time);
+
+/* innerLoop.h:35-36.  Top-B of |d_samp|^2 over samp_size entries, ascending indices into
+ * device int d_output[B]; adds seconds to *CTime. */
+int samples_indices(complex_t *d_samp, int B, int samp_size, int *d_output, double *CTime);
+
+/* innerLoop.h:38-41.  One loop: permute+filter into B buckets, FFT_B into device
+ * d_bins_f[B], top-B_thresh bucket indices into device d_J[B_thresh]. */
+int perm_filter_cutoff(complex_t *d_x, int n, complex_t *d_filter_time, int d_filter_size,
+                       int B_thresh, int B, int a, int ai, complex_t *d_bins_f, int *d_J,
+                       double *PF_T, double *BC_T);
+
+/* innerLoop.h:43-45.  Votes for one location loop into device int d_vote[n]; appends
+ * locations that reach loop_thresh to device d_I; *I_F is the host in/out hit count.
+ * d_J2 (device or host-readable, num_RLFC sorted distinct residues). */
+int reverse_hash(int *d_J, int n, int B_thresh, int B, int a, int ai, int loop_thresh,
+                 int *d_vote, int *d_I, int *I_F, double *G_T, int *d_J2, int num_RLFC,
+                 int W_RLFC, int RLFC_loops);
+
+/* innerLoop.h:47-48.  Medians over `loops` bucket rows; returns malloc'd host Node[*I_F]. */
+Node *estimate_values(int *d_I, int *I_F, complex_t *d_bins_f, int loops, int n, int *permute,
+                      int B, complex_t *d_filter_freq, int loops_loc);
+
+/* innerLoop.h:50 */
+int timesmod(const int x, const int a, const int n);
+
+/* filters.h:44-48.  Designs the filter (FP64) for d_x's size and runs the experiment
+ * (run_experiment) exactly as the reference's harness does. */
+void make_multiple_t(double lobefrac, double tolerance, int n, int b_f, complex_t *d_x, int B,
+                     int B_thresh, int loops_loc, int loops_thresh, int loops_est, int W_Comb,
+                     int Comb_loops, int repetitions, int *LARGE_FREQ, int k, complex_t *x_f,
+                     float t_H_to_D);
+
+/* experiment.h:32-46 */
+int cudafft_experiment(int n, double lobefrac, double tolerance, int b_f, int B, int B_thresh,
+                       int loops_loc, int loops_thresh, int loops_est, int W_RLFC,
+                       int RLFC_loops, int repetitions, int k, double std_noise);
+void run_experiment(complex_t *d_x, int n, complex_t *d_filter_time, complex_t *d_filter_freq,
+                    int d_filter_size, int B, int B_thresh, int loops_loc, int loops_thresh,
+                    int loops_est, int W_RLFC, int RLFC_loops, int repetitions,
+                    int *LARGE_FREQ, int k, complex_t *x_f, float t_H_to_D);
+void cuShift(complex_t *d_x, int n, int r);
+
+/* ---- native plan API --------------------------------------------------------------------- */
+typedef enum {
+  SFFT_OK = 0,
+  SFFT_ERR_INVALID = 1,        /* parameters violate the reference's asserts or ABI limits */
+  SFFT_ERR_ALLOC = 2,
+  SFFT_ERR_HIP = 3,
+  SFFT_ERR_HIT_OVERFLOW = 4,   /* more located indices than hit_capacity (reference: UB) */
+  SFFT_ERR_NO_FILTER = 5,
+  SFFT_ERR_NO_DEVICE = 6
+} sfft_status;
+
+typedef enum { SFFT_FP64 = 0, SFFT_FP32 = 1 } sfft_precision;
+
+typedef enum {
+  SFFT_FOLD_REFERENCE = 0,     /* n < 2^27: tiled fold + the reference's tail taps */
+  SFFT_FOLD_ALL_TAPS = 1       /* every tap i < w into bucket i mod B */
+} sfft_fold_mode;
+
+typedef struct {
+  int n;              /* signal length, power of two, 2^4 .. 2^30 */
+  int B;              /* buckets, power of two */
+  int B_thresh;       /* cutoff size (2k) */
+  int W_RLFC;         /* comb length, power of two */
+  int RLFC_loops;     /* comb repetitions */
+  int loops_thresh;   /* votes needed */
+  int loops_loc;      /* location loops */
+  int loops;          /* loops_loc + loops_est */
+  int w;              /* filter taps */
+  int fold_mode;      /* sfft_fold_mode */
+  int hit_capacity;   /* 0 -> MAXSIZE_HITS */
+} sfft_params;
+
+typedef struct sfft_plan sfft_plan;
+
+/* draws for one transform: a[l] odd, ai[l] = a[l]^-1 mod n (l < loops), comb offsets
+ * offsets[r] in [0, n/W_RLFC) (r < RLFC_loops).  Host arrays. */
+typedef struct {
+  const int *a;
+  const int *ai;
+  const int *offsets;
+} sfft_draws;
+
+const char *sfft_status_string(sfft_status s);
+/* validates against the reference's asserts (experiment.c:77,82-84; cuExperiment.cu:283) */
+sfft_status sfft_params_validate(const sfft_params *p);
+/* stream: a hipStream_t (NULL = default stream) */
+sfft_status sfft_plan_create(const sfft_params *p, sfft_precision prec, void *stream,
+                             sfft_plan **out);
+void sfft_plan_destroy(sfft_plan *plan);
+size_t sfft_plan_workspace_bytes(const sfft_plan *plan);
+
+/* filter: filter_time device pointer of w taps (complex128 for SFFT_FP64, complex64 for
+ * SFFT_FP32; the plan keeps the pointer, caller keeps ownership), and the m = n/B FP64
+ * frequency-response entries win[j] = filter_freq[(j - m/2) mod n] (host pointer, copied). */
+sfft_status sfft_plan_set_filter(sfft_plan *plan, const void *d_filter_time,
+                                 const complex_t *h_filter_win);
+
+/* One transform of device signal d_x (complex128[n] or complex64[n]).  Results stay on the
+ * device: nodes in d_nodes (Node[hit_capacity], device) and the count in *d_count (device
+ * int).  Asynchronous on the plan's stream. */
+sfft_status sfft_execute_async(sfft_plan *plan, const void *d_x, const sfft_draws *draws,
+                               Node *d_nodes, int *d_count);
+/* Same, then copies the count and nodes to host memory (h_nodes has hit_capacity entries)
+ * and synchronises the stream.  Returns SFFT_ERR_HIT_OVERFLOW when the hits exceed capacity. */
+sfft_status sfft_execute(sfft_plan *plan, const void *d_x, const sfft_draws *draws,
+                         Node *h_nodes, int *h_count);
+
+/* Stage timing with hipEvents on the plan stream (off by default).  After a completed
+ * transform, sfft_plan_stage_times writes seconds in the reference's table order
+ * (outerLoop.cu:174-187): votetable, restr-loc-index, perm+filter, reverse-hash,
+ * estimation, FFT+Cutoff, total. */
+sfft_status sfft_plan_set_profiling(sfft_plan *plan, int on);
+sfft_status sfft_plan_stage_times(sfft_plan *plan, double *seconds7);
+
+/* Debug/parity taps into the last transform (device pointers owned by the plan):
+ * 0 bins_f (loops x B complex), 1 J (loops_loc x B_thresh int), 2 J2 lists (RLFC_loops x
+ * B_thresh int), 3 hits (int, count from the last execute), 4 bins_t (loops x B complex),
+ * 5 nodes (Node[hit_capacity]), 6 count (int). */
+const void *sfft_plan_debug_buffer(const sfft_plan *plan, int which);
+
+/* ---- canonical FFT on the device (the same kernels the transform uses) ------------------ */
+/* In-place-or-not batched power-of-two FFT: count transforms of 2^log2n points, contiguous,
+ * complex128 (SFFT_FP64) or complex64 (SFFT_FP32).  Unnormalised, forward = exp(-2 pi i). */
+sfft_status sfft_fft_device(const void *d_in, void *d_out, int log2n, int count, int inverse,
+                            sfft_precision prec, void *stream);
+
+/* canonical twiddle exp(-+2 pi i e / T) (T power of two), as used by every FFT table */
+void sfft_twiddle(uint64_t e, uint64_t T, int inverse, double *re, double *im);
+/* pass split of a 2^log2n FFT; returns the number of passes */
+int sfft_fft_plan_passes(int log2n, int *pass_logs);
+
+/* ---- library-wide ------------------------------------------------------------------------ */
+const char *sfft_version(void);
+int sfft_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SFFT_H */

@@ -1,0 +1,165 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on identical inputs.
+
+Plain FP64 path: bit-exact (bucket spectra, cutoffs, restriction lists, hit list in order,
+node values).  Filters, permutations and comb offsets are computed once and fed to both.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import CFG1, SMALL_CONFIGS, bits, draws, oracle_filter, params, sparse_signal
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import sfft_amd
+    return sfft_amd
+
+
+def _dev(a, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.to("cuda") if dtype is None else t.to(dtype).to("cuda")
+
+
+# ---- canonical FFT ----------------------------------------------------------------------------
+@pytest.mark.parametrize("log2n", list(range(1, 18)) + [20, 22])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_fft64_bitexact(S, log2n, inverse):
+    n = 1 << log2n
+    count = 3 if n <= 1 << 17 else 1
+    rng = np.random.default_rng(log2n)
+    x = rng.standard_normal((count, n)) + 1j * rng.standard_normal((count, n))
+    got = S.fft(_dev(x), inverse=inverse).cpu().numpy()
+    for c in range(count):
+        want = O.fft64(x[c], inverse)
+        assert np.array_equal(bits(got[c]), bits(want)), f"row {c}"
+    ref = np.fft.ifft(x[0]) * n if inverse else np.fft.fft(x[0])
+    assert np.max(np.abs(got[0] - ref)) <= 1e-13 * np.max(np.abs(ref)) * (log2n + 1)
+
+
+@pytest.mark.parametrize("log2n", [1, 2, 5, 9, 11, 12, 13, 17, 22])
+def test_fft32_bitexact(S, log2n):
+    n = 1 << log2n
+    rng = np.random.default_rng(100 + log2n)
+    x = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))).astype(np.complex64)
+    got = S.fft(_dev(x)).cpu().numpy()
+    for c in range(2):
+        assert np.array_equal(bits(got[c]), bits(O.fft32(x[c])))
+
+
+# ---- the whole transform, FP64 plain path -----------------------------------------------------
+def _run_both(S, cfg, seed, fold_mode=O.FOLD_REFERENCE, snr_db=None, prm_override=None):
+    prm = params(cfg)
+    if prm_override:
+        prm.update(prm_override)
+    n, B, K = prm["n"], prm["B"], prm["B_thresh"]
+    x, locs, vals = sparse_signal(n, prm["k"], seed, snr_db)
+    ft, ff = oracle_filter(prm)
+    win = O.filter_window_slice(ff, n, B)
+    a, ai, offs = draws(n, prm["loops"], prm["RLFC_loops"], prm["W_RLFC"], seed + 7)
+    ref = O.outer_loop64(prm, x, ft, win, a, ai, offs, fold_mode=fold_mode, want_bins=True,
+                         want_J=True)
+    plan = S.Plan(prm, "fp64", fold_mode=fold_mode)
+    plan.set_filter(_dev(ft), win)
+    nodes, cnt = plan.execute(_dev(x), a, ai, offs)
+    return prm, plan, ref, nodes, cnt, locs
+
+
+def _assert_transform_equal(prm, plan, ref, nodes, cnt):
+    L, Ll, Rl, B, K = prm["loops"], prm["loops_loc"], prm["RLFC_loops"], prm["B"], prm["B_thresh"]
+    # comb restriction lists (per loop, ascending) -> merged set equals the oracle's J2
+    J2 = plan.debug_array(2, Rl * K, np.int32)
+    assert np.array_equal(np.unique(J2), ref["J2"])
+    bins_f = plan.debug_array(0, L * B, np.complex128).reshape(L, B)
+    assert np.array_equal(bits(bins_f), bits(ref["bins_f"]))
+    J = plan.debug_array(1, Ll * K, np.int32).reshape(Ll, K)
+    assert np.array_equal(J, ref["J"])
+    assert cnt == ref["I_F"]
+    hits = plan.debug_array(3, cnt, np.int32)
+    assert np.array_equal(hits, ref["hits"])          # same deterministic order
+    assert np.array_equal(nodes["key"], ref["nodes"]["key"])
+    assert np.array_equal(bits(nodes["re"]), bits(ref["nodes"]["re"]))
+    assert np.array_equal(bits(nodes["im"]), bits(ref["nodes"]["im"]))
+
+
+@pytest.mark.parametrize("cfg", SMALL_CONFIGS)
+def test_transform_small_bitexact(S, cfg):
+    prm, plan, ref, nodes, cnt, locs = _run_both(S, cfg, seed=cfg[0] * 31 + cfg[1])
+    _assert_transform_equal(prm, plan, ref, nodes, cnt)
+    # these small parameter sets are not the reference's tuned ones: recovery is checked
+    # loosely here (parity above is the point); tuned sets are checked exactly elsewhere
+    missed = len(set(locs.tolist()) - set(nodes["key"].tolist()))
+    assert missed <= max(1, len(locs) // 10)
+
+
+@pytest.mark.parametrize("cfg", SMALL_CONFIGS[:2])
+def test_transform_all_taps_bitexact(S, cfg):
+    prm, plan, ref, nodes, cnt, _ = _run_both(S, cfg, seed=5, fold_mode=O.FOLD_ALL_TAPS)
+    _assert_transform_equal(prm, plan, ref, nodes, cnt)
+
+
+def test_transform_noisy_bitexact(S):
+    prm, plan, ref, nodes, cnt, _ = _run_both(S, SMALL_CONFIGS[2], seed=11, snr_db=20.0)
+    _assert_transform_equal(prm, plan, ref, nodes, cnt)
+
+
+def test_transform_cfg1_bitexact(S):
+    """n=2^20, k=1000 (the reference's CPU-runnable configuration)."""
+    prm, plan, ref, nodes, cnt, locs = _run_both(S, CFG1, seed=2024)
+    _assert_transform_equal(prm, plan, ref, nodes, cnt)
+    assert set(locs.tolist()) <= set(nodes["key"].tolist())
+
+
+def test_transform_deterministic(S):
+    """two runs of the same plan give identical bits (no racy folds, ordered hit list)."""
+    prm = params(SMALL_CONFIGS[1])
+    n = prm["n"]
+    x, _, _ = sparse_signal(n, prm["k"], 3)
+    ft, ff = oracle_filter(prm)
+    win = O.filter_window_slice(ff, n, prm["B"])
+    a, ai, offs = draws(n, prm["loops"], prm["RLFC_loops"], prm["W_RLFC"], 9)
+    plan = S.Plan(prm, "fp64")
+    plan.set_filter(_dev(ft), win)
+    dx = _dev(x)
+    n1, c1 = plan.execute(dx, a, ai, offs)
+    n2, c2 = plan.execute(dx, a, ai, offs)
+    assert c1 == c2 and n1.tobytes() == n2.tobytes()
+
+
+def test_hit_overflow_is_reported(S):
+    prm = params(SMALL_CONFIGS[1])
+    n = prm["n"]
+    x, _, _ = sparse_signal(n, prm["k"], 3)
+    ft, ff = oracle_filter(prm)
+    win = O.filter_window_slice(ff, n, prm["B"])
+    a, ai, offs = draws(n, prm["loops"], prm["RLFC_loops"], prm["W_RLFC"], 9)
+    plan = S.Plan(prm, "fp64", hit_capacity=4)
+    plan.set_filter(_dev(ft), win)
+    with pytest.raises(S.SfftError) as e:
+        plan.execute(_dev(x), a, ai, offs)
+    assert e.value.status == 4
+
+
+# ---- filter design -------------------------------------------------------------------------
+@pytest.mark.parametrize("cfg", SMALL_CONFIGS[:3])
+def test_filter_design_bitexact(S, cfg):
+    prm = params(cfg)
+    ft_ref, ff_ref = oracle_filter(prm)
+    ft, ff = S.make_filter(prm["lobefrac"], prm["tolerance"], prm["n"], prm["b_f"], prm["w"])
+    assert np.array_equal(bits(ft.cpu().numpy()), bits(ft_ref))
+    assert np.array_equal(bits(ff), bits(ff_ref))
+
+
+def test_dft_any_matches_numpy(S):
+    for N in (3, 5, 7, 15, 123, 1001, 4095):
+        rng = np.random.default_rng(N)
+        x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+        got = S.dft_any(_dev(x)).cpu().numpy()
+        assert np.array_equal(bits(got), bits(O.dft_any64(x)))
+        ref = np.fft.fft(x)
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
