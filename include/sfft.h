@@ -196,6 +196,42 @@ void sfft_twiddle(uint64_t e, uint64_t T, int inverse, double *re, double *im);
 /* pass split of a 2^log2n FFT; returns the number of passes */
 int sfft_fft_plan_passes(int log2n, int *pass_logs);
 
+/* arbitrary-length FP64 DFT in place (Bluestein over the canonical FFT; cuFFT's role for the
+ * odd window lengths of cuExperiment.cu:305-308).  Synchronous. */
+sfft_status sfft_dft_any_device(complex_t *d_data, int N, int inverse, void *stream);
+
+/* ---- FP64 filter design (make_multiple_t without the experiment) ------------------------- */
+/* taps w of the Dolph-Chebyshev window (cuExperiment.cu:279-284); *t0 optional */
+int sfft_filter_taps(double lobefrac, double tolerance, double *t0);
+/* filter time taps into device d_filter_time[w] and the n-entry frequency response into host
+ * h_filter_freq[n] (may be NULL), exactly the computation of cuExperiment.cu:279-447
+ * (including the reference's NaN->0 Chebyshev rule, the dropped fold tap and the sequential
+ * boxcar and phase-ramp recurrences).  Synchronous. */
+sfft_status sfft_make_filter(double lobefrac, double tolerance, int n, int b_f, int w,
+                             complex_t *d_filter_time, complex_t *h_filter_freq, void *stream);
+/* win[j] = filter_freq[(j - m/2) mod n], j < m = n/B: the entries estimation reads */
+void sfft_filter_window(const complex_t *filter_freq, int n, int B, complex_t *win);
+sfft_status sfft_convert_c128_to_c64(const void *d_in, void *d_out, size_t count, void *stream);
+sfft_status sfft_convert_c64_to_c128(const void *d_in, void *d_out, size_t count, void *stream);
+
+/* ---- synthetic signals and the experiment metric (cuExperiment.cu:64-137, 181-221) ------ */
+/* k distinct locations (ascending) and unit-magnitude uniform-phase values from a seed */
+sfft_status sfft_signal_spectrum(int n, int k, uint64_t seed, int *h_locs, complex_t *h_vals);
+/* x = unnormalised inverse DFT of the sparse spectrum, plus complex Gaussian noise of
+ * per-component std sqrt(k / (2 * 10^(snr_db/10))) when snr_db > 0; x is complex128[n] or,
+ * with x_fp32, complex64[n].  Synchronous. */
+sfft_status sfft_signal_synthesize(int n, int k, const int *h_locs, const complex_t *h_vals,
+                                   double snr_db, uint64_t noise_seed, int x_fp32, void *d_x,
+                                   void *stream);
+/* X = FFT_n(x) / n in FP64 (device complex128[n]).  Synchronous. */
+sfft_status sfft_truth_spectrum(const void *d_x, int n, int x_fp32, complex_t *d_X, void *stream);
+/* run_experiment's metric against the truth spectrum d_X: MISSED (estimation, result) and
+ * err_total = sum over all n of |yhat - X|^2 ("MSE*k"); err_support and max_abs restricted
+ * to the kept support and the true locations.  d_X is restored before returning. */
+sfft_status sfft_metric(const Node *nodes, int I_F, int k, const int *locs, complex_t *d_X,
+                        int n, int *missed_est, int *missed_res, double *err_total,
+                        double *err_support, double *max_abs, void *stream);
+
 /* ---- library-wide ------------------------------------------------------------------------ */
 const char *sfft_version(void);
 int sfft_device_count(void);
