@@ -172,7 +172,7 @@ static int cutoff_device(int fp32, const void* d_keys, int N, int K, int* d_out,
   (void)hipMemsetAsync(w, 0, select_work_bytes(1, N), st);
   SelectWork sw;
   select_work_bind(&sw, w, 1, N);
-  const hipError_t e = select_topk(fp32, d_keys, N, N, K, 1, d_out, sw, st);
+  const hipError_t e = select_topk(fp32, d_keys, N, N, K, 1, d_out, 1, sw, st);
   (void)hipStreamSynchronize(st);
   (void)hipFree(w);
   return e == hipSuccess ? 0 : -1;

@@ -1,0 +1,476 @@
+// sfft_fft.hip -- hand-written CDNA4 (gfx950) kernels of the sparse-FFT hot path, part 1.
+//
+//  * fft_pass_kernel      canonical multi-pass big-radix Stockham FFT (LDS sub-FFT, radix-4/2
+//                         stages), optional fused comb gather (RLFC) and |X|^2 key epilogue
+//  * perm_filter_kernel   permutation + filtering: bucket-owner gather fold, all loops of a
+//                         transform in one launch, fixed (deterministic) tap order
+//
+// Every floating-point operation that must match the CPU oracle bit for bit is written out
+// explicitly (fma where fused, plain ops elsewhere); the library is built with
+// -ffp-contract=off so the compiler adds no fusion of its own.
+#include "sfft_internal.h"
+
+namespace sfft {
+
+template <typename R>
+struct CT;
+template <>
+struct CT<double> {
+  using T2 = double2;
+  using K = unsigned long long;
+};
+template <>
+struct CT<float> {
+  using T2 = float2;
+  using K = unsigned int;
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 w) {
+  return make_double2(fma(a.x, w.x, -(a.y * w.y)), fma(a.x, w.y, a.y * w.x));
+}
+__device__ __forceinline__ float2 cmul(float2 a, float2 w) {
+  return make_float2(fmaf(a.x, w.x, -(a.y * w.y)), fmaf(a.x, w.y, a.y * w.x));
+}
+__device__ __forceinline__ unsigned long long sample_key(double2 v) {
+  return (unsigned long long)__double_as_longlong(fma(v.x, v.x, v.y * v.y));
+}
+__device__ __forceinline__ unsigned int sample_key(float2 v) {
+  return __float_as_uint(fmaf(v.x, v.x, v.y * v.y));
+}
+
+template <typename T2>
+__device__ __forceinline__ T2 cadd(T2 a, T2 b) {
+  T2 r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+template <typename T2>
+__device__ __forceinline__ T2 csub(T2 a, T2 b) {
+  T2 r;
+  r.x = a.x - b.x;
+  r.y = a.y - b.y;
+  return r;
+}
+
+// radix-4 butterfly: t0=a0+a2, t1=a0-a2, t2=a1+a3, t3=(a1-a3)*(-+i); y=(t0+t2, t1+t3, t0-t2, t1-t3)
+template <typename T2>
+__device__ __forceinline__ void bfly4(T2& a0, T2& a1, T2& a2, T2& a3, int inverse) {
+  const T2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), d = csub(a1, a3);
+  T2 t3;
+  if (inverse) {
+    t3.x = -d.y;
+    t3.y = d.x;
+  } else {
+    t3.x = d.y;
+    t3.y = -d.x;
+  }
+  a0 = cadd(t0, t2);
+  a1 = cadd(t1, t3);
+  a2 = csub(t0, t2);
+  a3 = csub(t1, t3);
+}
+
+// ============================================================================================
+// FFT pass
+// ============================================================================================
+constexpr int FFT_NT = 512;
+constexpr int FFT_CAP = 8192;                // complex elements per tile (LDS)
+constexpr int FFT_EPT = FFT_CAP / FFT_NT;    // elements per thread
+constexpr int FFT_QMAX = 16;                 // columns per tile
+
+struct PassArgs {
+  const void* in;
+  long long in_bstride;
+  void* out;
+  long long out_bstride;
+  void* keys;
+  long long keys_bstride;
+  int key_batches;
+  int log2N, r, log2Lprev, log2Q;
+  int inverse;
+  const void* twR;
+  const void* tdir;
+  const void* thi;
+  const void* tlo;
+  int s;
+  const void* comb_x;
+  const int* comb_offs;
+  int comb_sigma;
+};
+
+template <typename T2>
+__device__ __forceinline__ T2 outer_tw(const PassArgs& A, uint32_t e, int log2Lcur) {
+  if (log2Lcur <= kMaxPassLog) return ((const T2*)A.tdir)[e];
+  const uint32_t lo = e & ((1u << A.s) - 1u), hi = e >> A.s;
+  return cmul(((const T2*)A.thi)[hi], ((const T2*)A.tlo)[lo]);
+}
+
+// LDS layout of a tile: column c at lds[c * RP + LPAD(v)] with one padding slot after every 16
+// elements and a one-slot skew between columns, so the radix-4 stage's stride-4 and stride-16
+// stores and the column-interleaved loads spread over the LDS banks.
+#define LPAD(x) ((x) + ((x) >> 4))
+
+template <typename T2>
+__device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol, int log2ns,
+                                           const T2* __restrict__ twR, int inverse) {
+  const int tid = threadIdx.x;
+  const int Rq = 1 << (r - 2);
+  const int nbf = (1 << log2ncol) * Rq;
+  const int ns = 1 << log2ns;
+  const int tstep = 1 << (r - log2ns - 2);
+  T2 v[FFT_EPT];
+#pragma unroll
+  for (int kk = 0; kk < FFT_EPT / 4; ++kk) {
+    const int bi = tid + kk * FFT_NT;
+    if (bi < nbf) {
+      const int c = bi >> (r - 2);
+      const int j = bi & (Rq - 1);
+      const T2* col = lds + c * RP;
+      T2 a0 = col[LPAD(j)], a1 = col[LPAD(j + Rq)], a2 = col[LPAD(j + 2 * Rq)],
+         a3 = col[LPAD(j + 3 * Rq)];
+      if (ns > 1) {
+        const int jm = j & (ns - 1);
+        a1 = cmul(a1, twR[jm * tstep]);
+        a2 = cmul(a2, twR[2 * jm * tstep]);
+        a3 = cmul(a3, twR[3 * jm * tstep]);
+      }
+      bfly4(a0, a1, a2, a3, inverse);
+      v[4 * kk + 0] = a0;
+      v[4 * kk + 1] = a1;
+      v[4 * kk + 2] = a2;
+      v[4 * kk + 3] = a3;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int kk = 0; kk < FFT_EPT / 4; ++kk) {
+    const int bi = tid + kk * FFT_NT;
+    if (bi < nbf) {
+      const int c = bi >> (r - 2);
+      const int j = bi & (Rq - 1);
+      T2* col = lds + c * RP;
+      const int base = ((j >> log2ns) << (log2ns + 2)) + (j & (ns - 1));
+      col[LPAD(base)] = v[4 * kk + 0];
+      col[LPAD(base + ns)] = v[4 * kk + 1];
+      col[LPAD(base + 2 * ns)] = v[4 * kk + 2];
+      col[LPAD(base + 3 * ns)] = v[4 * kk + 3];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T2>
+__device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol, int log2ns,
+                                           const T2* __restrict__ twR) {
+  const int tid = threadIdx.x;
+  const int Rh = 1 << (r - 1);
+  const int nbf = (1 << log2ncol) * Rh;
+  const int ns = 1 << log2ns;
+  const int tstep = 1 << (r - log2ns - 1);
+  T2 v[FFT_EPT];
+#pragma unroll
+  for (int kk = 0; kk < FFT_EPT / 2; ++kk) {
+    const int bi = tid + kk * FFT_NT;
+    if (bi < nbf) {
+      const int c = bi >> (r - 1);
+      const int j = bi & (Rh - 1);
+      const T2* col = lds + c * RP;
+      T2 a0 = col[LPAD(j)], a1 = col[LPAD(j + Rh)];
+      if (ns > 1) a1 = cmul(a1, twR[(j & (ns - 1)) * tstep]);
+      v[2 * kk + 0] = cadd(a0, a1);
+      v[2 * kk + 1] = csub(a0, a1);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int kk = 0; kk < FFT_EPT / 2; ++kk) {
+    const int bi = tid + kk * FFT_NT;
+    if (bi < nbf) {
+      const int c = bi >> (r - 1);
+      const int j = bi & (Rh - 1);
+      T2* col = lds + c * RP;
+      const int base = ((j >> log2ns) << (log2ns + 1)) + (j & (ns - 1));
+      col[LPAD(base)] = v[2 * kk + 0];
+      col[LPAD(base + ns)] = v[2 * kk + 1];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename Real, int LOAD, int STORE>
+__global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
+  using T2 = typename CT<Real>::T2;
+  using KT = typename CT<Real>::K;
+  __shared__ T2 lds[FFT_CAP + FFT_CAP / 16 + FFT_QMAX];
+  const int tid = threadIdx.x;
+  const long long b = blockIdx.y;
+  const int r = A.r;
+  const int R = 1 << r;
+  const int RP = R + (R >> 4) + 1;   // padded column stride (see LPAD)
+  const int log2Lprev = A.log2Lprev;
+  const int log2Lcur = log2Lprev + r;
+  const long long Lprev = 1LL << log2Lprev;
+  const long long nq = 1LL << (A.log2N - log2Lcur);
+  const bool first = (log2Lprev == 0);
+  int log2ncol;
+  long long q0 = 0, q = 0, t0 = 0;
+  if (first) {
+    log2ncol = min(A.log2Q, A.log2N - log2Lcur);
+    q0 = (long long)blockIdx.x << log2ncol;
+  } else {
+    log2ncol = A.log2Q;
+    const int log2tpq = log2Lprev - A.log2Q;
+    q = (long long)blockIdx.x >> log2tpq;
+    t0 = ((long long)blockIdx.x & ((1LL << log2tpq) - 1)) << A.log2Q;
+  }
+  const int ncol = 1 << log2ncol;
+  const int TE = ncol << r;
+
+  // ---- load (+ outer twiddle) into LDS, column c at lds[c*RP + v]
+  const T2* __restrict__ in = (const T2*)A.in + b * A.in_bstride;
+#pragma unroll
+  for (int k = 0; k < FFT_EPT; ++k) {
+    const int e = tid + k * FFT_NT;
+    if (e < TE) {
+      const int c = e & (ncol - 1);
+      const int v = e >> log2ncol;
+      long long g;
+      if (first)
+        g = q0 + c + nq * v;
+      else
+        g = (q + nq * v) * Lprev + t0 + c;
+      T2 val;
+      if (LOAD == kLoadComb)
+        val = ((const T2*)A.comb_x)[(long long)A.comb_offs[b] + g * (long long)A.comb_sigma];
+      else
+        val = in[g];
+      if (!first && v > 0) val = cmul(val, outer_tw<T2>(A, (uint32_t)((t0 + c) * v), log2Lcur));
+      lds[c * RP + LPAD(v)] = val;
+    }
+  }
+  __syncthreads();
+
+  // ---- sub-FFT of every column: radix-4 stages, then one radix-2 stage if r is odd
+  const T2* __restrict__ twR = (const T2*)A.twR;
+  int log2ns = 0;
+  int rem = r;
+  while (rem >= 2) {
+    fft_stage4<T2>(lds, RP, r, log2ncol, log2ns, twR, A.inverse);
+    log2ns += 2;
+    rem -= 2;
+  }
+  if (rem == 1) fft_stage2<T2>(lds, RP, r, log2ncol, log2ns, twR);
+
+  // ---- store: out[q*Lcur + t + Lprev*k]
+  T2* __restrict__ out = (T2*)A.out + b * A.out_bstride;
+  KT* __restrict__ keys = (KT*)A.keys + b * A.keys_bstride;
+  const bool wkeys = (STORE != kStoreCplx) && (b < A.key_batches);
+#pragma unroll
+  for (int k = 0; k < FFT_EPT; ++k) {
+    const int e = tid + k * FFT_NT;
+    if (e < TE) {
+      int c, kk;
+      long long g;
+      if (first) {
+        c = e >> r;
+        kk = e & (R - 1);
+        g = (q0 + c) * R + kk;
+      } else {
+        c = e & (ncol - 1);
+        kk = e >> log2ncol;
+        g = (q << log2Lcur) + t0 + c + Lprev * kk;
+      }
+      const T2 val = lds[c * RP + LPAD(kk)];
+      if (STORE != kStoreKeys) out[g] = val;
+      if (wkeys) keys[g] = sample_key(val);
+    }
+  }
+}
+
+template <typename Real, int LOAD, int STORE>
+static hipError_t launch_pass(const PassArgs& A, int tiles, int batch, hipStream_t st) {
+  hipLaunchKernelGGL((fft_pass_kernel<Real, LOAD, STORE>), dim3(tiles, batch), dim3(FFT_NT), 0,
+                     st, A);
+  return hipGetLastError();
+}
+
+template <typename Real>
+static hipError_t dispatch_pass(const PassArgs& A, int load, int store, int tiles, int batch,
+                                hipStream_t st) {
+  if (load == kLoadComb) {
+    if (store == kStoreCplx) return launch_pass<Real, kLoadComb, kStoreCplx>(A, tiles, batch, st);
+    if (store == kStoreKeys) return launch_pass<Real, kLoadComb, kStoreKeys>(A, tiles, batch, st);
+    return launch_pass<Real, kLoadComb, kStoreBoth>(A, tiles, batch, st);
+  }
+  if (store == kStoreCplx) return launch_pass<Real, kLoadPlain, kStoreCplx>(A, tiles, batch, st);
+  if (store == kStoreKeys) return launch_pass<Real, kLoadPlain, kStoreKeys>(A, tiles, batch, st);
+  return launch_pass<Real, kLoadPlain, kStoreBoth>(A, tiles, batch, st);
+}
+
+hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
+  const FftTables& T = *L.tab;
+  const int log2N = T.log2n;
+  const long long N = 1LL << log2N;
+  const int np = T.npass;
+  if (np == 0) {
+    // N == 1: the DFT is the identity (copy / keys)
+    return hipErrorInvalidValue;
+  }
+  int log2Lprev = 0;
+  const void* cur_in = L.in;
+  long long cur_in_stride = L.in_bstride;
+  for (int p = 0; p < np; ++p) {
+    const int r = T.logs[p];
+    const int log2Lcur = log2Lprev + r;
+    const bool last = (p == np - 1);
+    // pass p writes to out when (np - 1 - p) is even, else to scratch
+    void* dst = (((np - 1 - p) & 1) == 0) ? L.out : L.scratch;
+    const long long dst_stride = (dst == L.out) ? L.out_bstride : N;
+    PassArgs A;
+    A.in = cur_in;
+    A.in_bstride = cur_in_stride;
+    A.out = dst;
+    A.out_bstride = dst_stride;
+    A.keys = L.keys;
+    A.keys_bstride = L.keys_bstride;
+    A.key_batches = L.key_batches;
+    A.log2N = log2N;
+    A.r = r;
+    A.log2Lprev = log2Lprev;
+    int log2Q = 0;
+    while ((1 << (log2Q + 1)) <= FFT_QMAX && ((1 << (log2Q + 1)) << r) <= FFT_CAP) ++log2Q;
+    int tiles;
+    if (p == 0) {
+      const int lnq = log2N - log2Lcur;
+      const int lq = log2Q < lnq ? log2Q : lnq;
+      tiles = 1 << (lnq - lq);
+    } else {
+      if (log2Q > log2Lprev) log2Q = log2Lprev;
+      tiles = (int)((1LL << (log2N - log2Lcur)) << (log2Lprev - log2Q));
+    }
+    A.log2Q = log2Q;
+    A.inverse = T.inverse;
+    A.twR = T.twR[p];
+    A.tdir = T.tdir[p];
+    A.thi = T.thi[p];
+    A.tlo = T.tlo[p];
+    A.s = T.s[p];
+    A.comb_x = L.comb_x;
+    A.comb_offs = L.comb_offs;
+    A.comb_sigma = L.comb_sigma;
+    const int load = (p == 0) ? L.load : kLoadPlain;
+    const int store = last ? L.store : kStoreCplx;
+    hipError_t e = T.is_fp32 ? dispatch_pass<float>(A, load, store, tiles, L.batch, st)
+                             : dispatch_pass<double>(A, load, store, tiles, L.batch, st);
+    if (e != hipSuccess) return e;
+    cur_in = dst;
+    cur_in_stride = dst_stride;
+    log2Lprev = log2Lcur;
+  }
+  return hipSuccess;
+}
+
+// ============================================================================================
+// permutation + filtering (bucket-owner fold)
+// ============================================================================================
+struct Acc64 {
+  double2 s;
+  __device__ __forceinline__ void init() { s = make_double2(0.0, 0.0); }
+  __device__ __forceinline__ void add(double2 x, double2 f) {
+    s.x = fma(x.x, f.x, s.x);
+    s.x = fma(-x.y, f.y, s.x);
+    s.y = fma(x.x, f.y, s.y);
+    s.y = fma(x.y, f.x, s.y);
+  }
+  __device__ __forceinline__ double2 get() const { return s; }
+};
+
+struct Acc32 {  // Kahan-compensated FP32 fold
+  float sx, cx, sy, cy;
+  __device__ __forceinline__ void init() { sx = cx = sy = cy = 0.f; }
+  __device__ __forceinline__ static void kahan(float& sum, float& comp, float term) {
+    const float y = term - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+  }
+  __device__ __forceinline__ void add(float2 x, float2 f) {
+    const float tx = fmaf(x.x, f.x, -(x.y * f.y));
+    const float ty = fmaf(x.x, f.y, x.y * f.x);
+    kahan(sx, cx, tx);
+    kahan(sy, cy, ty);
+  }
+  __device__ __forceinline__ float2 get() const { return make_float2(sx, sy); }
+};
+
+template <typename Real>
+struct AccOf;
+template <>
+struct AccOf<double> {
+  using A = Acc64;
+};
+template <>
+struct AccOf<float> {
+  using A = Acc32;
+};
+
+constexpr int PF_NT = 256;
+
+template <typename Real, int LPT>
+__global__ __launch_bounds__(PF_NT) void perm_filter_kernel(
+    const typename CT<Real>::T2* __restrict__ x, const typename CT<Real>::T2* __restrict__ f,
+    typename CT<Real>::T2* __restrict__ bins, const int* __restrict__ ai_dev, uint32_t mask,
+    int B, int w, int loops, int tiled) {
+  using T2 = typename CT<Real>::T2;
+  using AC = typename AccOf<Real>::A;
+  const int r = blockIdx.x * PF_NT + threadIdx.x;
+  if (r >= B) return;
+  const int l0 = blockIdx.y * LPT;
+  uint32_t ai[LPT];
+  AC acc[LPT];
+#pragma unroll
+  for (int li = 0; li < LPT; ++li) {
+    ai[li] = (l0 + li < loops) ? (uint32_t)ai_dev[l0 + li] : 0u;
+    acc[li].init();
+  }
+  auto tap = [&](uint32_t id) {
+    const T2 fv = f[id];
+    T2 xv[LPT];
+#pragma unroll
+    for (int li = 0; li < LPT; ++li) xv[li] = x[(id * ai[li]) & mask];
+#pragma unroll
+    for (int li = 0; li < LPT; ++li) acc[li].add(xv[li], fv);
+  };
+  if (tiled) {
+    const int T = w / B;
+#pragma unroll 2
+    for (int t = 0; t < T; ++t) tap((uint32_t)(t * B + r));
+    if (r < w - T * B) tap((uint32_t)(r * T + B));
+  } else {
+#pragma unroll 2
+    for (uint32_t id = (uint32_t)r; id < (uint32_t)w; id += (uint32_t)B) tap(id);
+  }
+#pragma unroll
+  for (int li = 0; li < LPT; ++li)
+    if (l0 + li < loops) bins[(long long)(l0 + li) * B + r] = acc[li].get();
+}
+
+hipError_t perm_filter_launch(int is_fp32, const void* x, const void* f, void* bins,
+                              const int* ai_dev, int n, int B, int w, int loops, int tiled,
+                              hipStream_t st) {
+  constexpr int LPT = 3;
+  dim3 grid((B + PF_NT - 1) / PF_NT, (loops + LPT - 1) / LPT);
+  const uint32_t mask = (uint32_t)n - 1u;
+  if (is_fp32)
+    hipLaunchKernelGGL((perm_filter_kernel<float, LPT>), grid, dim3(PF_NT), 0, st,
+                       (const float2*)x, (const float2*)f, (float2*)bins, ai_dev, mask, B, w,
+                       loops, tiled);
+  else
+    hipLaunchKernelGGL((perm_filter_kernel<double, LPT>), grid, dim3(PF_NT), 0, st,
+                       (const double2*)x, (const double2*)f, (double2*)bins, ai_dev, mask, B, w,
+                       loops, tiled);
+  return hipGetLastError();
+}
+
+}  // namespace sfft

@@ -1,0 +1,380 @@
+// sfft_hash.hip -- hand-written CDNA4 (gfx950) kernels of the sparse-FFT hot path, part 2.
+//
+//  * hash_*_kernel        reverse hash: approved-residue bitmaps, packed per-loop vote masks,
+//                         deterministic hit list (loop, bucket, walk order)
+//  * estimate_kernel      per-location FP64 filter division and stable lower median
+//
+// Floating-point operations that must match the CPU oracle bit for bit are written out
+// explicitly (fma where fused); the library is built with -ffp-contract=off.
+#include "sfft_internal.h"
+
+namespace sfft {
+
+template <typename R>
+struct CT;
+template <>
+struct CT<double> {
+  using T2 = double2;
+};
+template <>
+struct CT<float> {
+  using T2 = float2;
+};
+
+// ============================================================================================
+// block helpers
+// ============================================================================================
+// exclusive scan over a block of up to 1024 threads (wave64)
+__device__ __forceinline__ int block_excl_scan(int v, int* sh) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[wid] = x;
+  __syncthreads();
+  int off = 0;
+  for (int i = 0; i < wid; ++i) off += sh[i];
+  __syncthreads();
+  return off + x - v;
+}
+
+__device__ __forceinline__ int block_sum(int v, int* sh) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  if (lane == 0) sh[wid] = x;
+  __syncthreads();
+  int tot = 0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int i = 0; i < nw; ++i) tot += sh[i];
+  __syncthreads();
+  return tot;
+}
+
+// ============================================================================================
+// reverse hash
+// ============================================================================================
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t words) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t s = (size_t)gridDim.x * blockDim.x;
+  for (; i < words; i += s) p[i] = v;
+}
+
+hipError_t fill_u32(void* p, uint32_t v, size_t words, hipStream_t st) {
+  if (words == 0) return hipSuccess;
+  size_t blocks = (words + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (uint32_t*)p, v,
+                     words);
+  return hipGetLastError();
+}
+
+hipError_t hash_clear_launch(const HashArgs& h, hipStream_t st) {
+  const size_t vote_words = ((size_t)h.n * h.bpe + 31) / 32;
+  hipError_t e = fill_u32(h.vote, 0u, vote_words, st);
+  if (e != hipSuccess) return e;
+  e = fill_u32(h.bitmap, 0u, (size_t)h.loops_loc * h.bm_words, st);
+  if (e != hipSuccess) return e;
+  return fill_u32(h.maxfirst, 0xFFFFFFFFu, (size_t)h.loops_loc, st);
+}
+
+__global__ void hash_bitmap_kernel(HashArgs h) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = blockIdx.y;
+  if (i >= h.nJ2) return;
+  const uint64_t j2 = (uint64_t)h.J2[i];
+  const uint32_t first = (uint32_t)((j2 * (uint64_t)(uint32_t)h.ai_dev[l]) & (uint64_t)(h.W - 1));
+  atomicOr(&h.bitmap[(size_t)l * h.bm_words + (first >> 5)], 1u << (first & 31));
+  atomicMax(&h.maxfirst[l], (int)first);
+}
+
+hipError_t hash_bitmap_launch(const HashArgs& h, hipStream_t st) {
+  dim3 grid((h.nJ2 + 255) / 256, h.loops_loc);
+  hipLaunchKernelGGL(hash_bitmap_kernel, grid, dim3(256), 0, st, h);
+  return hipGetLastError();
+}
+
+// Lanes per bucket window: one per 32 positions (a bitmap word), at most a wave, at least one.
+// m = n/B is a power of two, so the count is a power of two and divides 64.
+__host__ __device__ __forceinline__ int hash_lpj(long long m) {
+  long long l = m / 32;
+  if (l < 1) l = 1;
+  if (l > 64) l = 64;
+  return (int)l;
+}
+
+// Enumerate, in increasing circular order, the approved permuted positions p in the slice
+// [low + d_begin, low + d_begin + d_len) of bucket J's window [J m - m/2, J m + m/2) (mod n);
+// the whole window is empty when (low mod W) exceeds the largest approved residue (the
+// reference's findUpperBound returns the last entry and the walk stops without wrapping).
+// Calls fn(loc) with loc = p * a mod n.
+template <typename F>
+__device__ __forceinline__ void hash_walk(const HashArgs& h, int l, int i, long long d_begin,
+                                          long long d_len, F&& fn) {
+  const long long n = h.n;
+  const long long m = n / h.B;
+  const long long J = h.J[(long long)l * h.K + i];
+  long long low = J * m - (m >> 1);
+  low = (low + n) & (n - 1);
+  const int key = (int)(low & (h.W - 1));
+  if (key > h.maxfirst[l]) return;
+  const unsigned* bm = h.bitmap + (size_t)l * h.bm_words;
+  const uint64_t a = (uint64_t)(uint32_t)h.a_dev[l];
+  const uint64_t nmask = (uint64_t)n - 1;
+  const long long d_end = d_begin + d_len;
+  for (long long d = d_begin; d < d_end;) {
+    const uint64_t p = (uint64_t)((low + d) & (n - 1));
+    const uint32_t bpos = (uint32_t)(p & (uint64_t)(h.W - 1));
+    const int off = (int)(bpos & 31u);
+    const long long left = d_end - d;
+    const int take = (left < 32 - off) ? (int)left : (32 - off);
+    uint32_t word = bm[bpos >> 5] >> off;
+    if (take < 32) word &= (1u << take) - 1u;
+    while (word) {
+      const int bit = __ffs(word) - 1;
+      word &= word - 1u;
+      const uint64_t pp = p + (uint64_t)bit;
+      fn((uint32_t)((pp * a) & nmask));
+    }
+    d += take;
+  }
+}
+
+constexpr int HASH_NT = 256;
+
+int hash_blocks(int loops_loc, int K, long long m) {
+  return (int)(((long long)loops_loc * K * hash_lpj(m) + HASH_NT - 1) / HASH_NT);
+}
+
+// thread -> (location loop l, bucket index i, slice sub of lpj)
+struct HashSlot {
+  int l, i, sub, lpj;
+  long long jidx, d0, dl;
+  bool valid;
+};
+
+__device__ __forceinline__ HashSlot hash_slot(const HashArgs& h) {
+  HashSlot s;
+  const long long m = (long long)h.n / h.B;
+  s.lpj = hash_lpj(m);
+  const long long gid = (long long)blockIdx.x * HASH_NT + threadIdx.x;
+  s.jidx = gid / s.lpj;
+  s.sub = (int)(gid % s.lpj);
+  s.valid = s.jidx < (long long)h.loops_loc * h.K;
+  s.l = (int)(s.jidx / h.K);
+  s.i = (int)(s.jidx - (long long)s.l * h.K);
+  s.dl = m / s.lpj;
+  s.d0 = s.sub * s.dl;
+  return s;
+}
+
+__global__ __launch_bounds__(HASH_NT) void hash_vote_kernel(HashArgs h) {
+  const HashSlot s = hash_slot(h);
+  if (!s.valid) return;
+  const int lgepw = 5 - __ffs(h.bpe) + 1;  // log2(32 / bpe)
+  const uint32_t epw_mask = (1u << lgepw) - 1u;
+  hash_walk(h, s.l, s.i, s.d0, s.dl, [&](uint32_t loc) {
+    atomicOr(&h.vote[loc >> lgepw], (1u << s.l) << ((loc & epw_mask) * (uint32_t)h.bpe));
+  });
+}
+
+hipError_t hash_vote_launch(const HashArgs& h, hipStream_t st) {
+  const int nb = hash_blocks(h.loops_loc, h.K, (long long)h.n / h.B);
+  hipLaunchKernelGGL(hash_vote_kernel, dim3(nb), dim3(HASH_NT), 0, st, h);
+  return hipGetLastError();
+}
+
+// a location is a hit of loop l when it is a candidate of l and exactly loops_thresh of the
+// loops 0..l voted for it (the loop in which the reference's count reaches the threshold)
+__device__ __forceinline__ bool hash_is_hit(const HashArgs& h, int l, uint32_t loc, int lgepw,
+                                            uint32_t epw_mask) {
+  const uint32_t word = h.vote[loc >> lgepw];
+  const uint32_t mask = (word >> ((loc & epw_mask) * (uint32_t)h.bpe)) &
+                        (uint32_t)((1ull << h.bpe) - 1ull);
+  return __popc(mask & ((2u << l) - 1u)) == h.loops_thresh;
+}
+
+__device__ __forceinline__ int hash_count_slice(const HashArgs& h, const HashSlot& s) {
+  const int lgepw = 5 - __ffs(h.bpe) + 1;
+  const uint32_t epw_mask = (1u << lgepw) - 1u;
+  int c = 0;
+  if (s.valid)
+    hash_walk(h, s.l, s.i, s.d0, s.dl,
+              [&](uint32_t loc) { c += hash_is_hit(h, s.l, loc, lgepw, epw_mask); });
+  return c;
+}
+
+__global__ __launch_bounds__(HASH_NT) void hash_count_kernel(HashArgs h) {
+  __shared__ int sh[16];
+  const HashSlot s = hash_slot(h);
+  const int c = hash_count_slice(h, s);
+  int tot = c;   // total of the bucket's lanes (aligned groups of lpj lanes)
+  for (int o = 1; o < s.lpj; o <<= 1) tot += __shfl_xor(tot, o, 64);
+  const int v = (s.valid && s.sub == 0) ? tot : 0;
+  const int ex = block_excl_scan(v, sh);
+  if (s.valid && s.sub == 0) h.hitoff[s.jidx] = ex;
+  if (threadIdx.x == HASH_NT - 1) h.blocksum[blockIdx.x] = ex + v;
+}
+
+// exclusive scan of the per-block hit totals: 1024 threads x 8 consecutive blocks per round
+__global__ __launch_bounds__(1024) void hash_scan_kernel(HashArgs h, int nblocks) {
+  __shared__ int sh[16];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nblocks; base += 8192) {
+    const int i0 = base + threadIdx.x * 8;
+    int v[8], s = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      v[u] = (i0 + u < nblocks) ? h.blocksum[i0 + u] : 0;
+      s += v[u];
+    }
+    const int ex = block_excl_scan(s, sh);
+    const int c0 = carry;
+    int run = c0 + ex;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (i0 + u < nblocks) h.blockoff[i0 + u] = run;
+      run += v[u];
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = run;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *h.count = carry;
+}
+
+__global__ __launch_bounds__(HASH_NT) void hash_write_kernel(HashArgs h) {
+  const HashSlot s = hash_slot(h);
+  const int c = hash_count_slice(h, s);
+  // exclusive scan over the bucket's lanes (in window order)
+  const int lane = threadIdx.x & 63;
+  int x = c;
+  for (int o = 1; o < s.lpj; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if ((lane & (s.lpj - 1)) >= o) x += y;
+  }
+  if (!s.valid || c == 0) return;
+  const int lgepw = 5 - __ffs(h.bpe) + 1;
+  const uint32_t epw_mask = (1u << lgepw) - 1u;
+  int pos = h.blockoff[blockIdx.x] + h.hitoff[s.jidx] + (x - c);
+  hash_walk(h, s.l, s.i, s.d0, s.dl, [&](uint32_t loc) {
+    if (hash_is_hit(h, s.l, loc, lgepw, epw_mask)) {
+      if (pos < h.capacity) h.hits[pos] = (int)loc;
+      ++pos;
+    }
+  });
+}
+
+hipError_t hash_hits_launch(const HashArgs& h, hipStream_t st) {
+  const int nb = hash_blocks(h.loops_loc, h.K, (long long)h.n / h.B);
+  hipLaunchKernelGGL(hash_count_kernel, dim3(nb), dim3(HASH_NT), 0, st, h);
+  hipLaunchKernelGGL(hash_scan_kernel, dim3(1), dim3(1024), 0, st, h, nb);
+  hipLaunchKernelGGL(hash_write_kernel, dim3(nb), dim3(HASH_NT), 0, st, h);
+  return hipGetLastError();
+}
+
+// ============================================================================================
+// estimation
+// ============================================================================================
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(float2 v) {
+  return make_double2((double)v.x, (double)v.y);
+}
+
+template <typename Real, int MAXL>
+__global__ __launch_bounds__(256) void estimate_kernel(
+    const int* __restrict__ hits, const int* __restrict__ count, int capacity,
+    const typename CT<Real>::T2* __restrict__ bins, int loops, uint32_t n,
+    const int* __restrict__ ai_dev, int B, const double2* __restrict__ win,
+    Node* __restrict__ out) {
+  const int total = min(*count, capacity);
+  const long long m = (long long)(n / (uint32_t)B);
+  const long long nn = n;
+  const int med = (loops - 1) / 2;
+  for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < total; h += gridDim.x * blockDim.x) {
+    const uint32_t I = (uint32_t)hits[h];
+    double vr[MAXL], vi[MAXL];
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l) {
+      vr[l] = 0.0;
+      vi[l] = 0.0;
+      if (l < loops) {
+        const long long p = (long long)(((uint64_t)(uint32_t)ai_dev[l] * I) & (uint64_t)(n - 1));
+        long long hashed = p / m;
+        long long dist = p & (m - 1);
+        if (dist > m / 2) {
+          hashed = (hashed + 1) & (B - 1);
+          dist -= m;
+        }
+        dist = (nn - dist) & (nn - 1);
+        const double2 F = win[(dist + m / 2) & (nn - 1)];
+        const double2 bb = to_d2(bins[(long long)l * B + hashed]);
+        const double abs2 = fma(F.x, F.x, F.y * F.y);
+        const double mr = fma(bb.x, F.x, bb.y * F.y);
+        const double mi = fma(bb.y, F.x, -(bb.x * F.y));
+        vr[l] = mr / abs2;
+        vi[l] = mi / abs2;
+      }
+    }
+    // stable rank selection: position of j in an insertion sort = #(v_i < v_j) + #(i<j, v_i==v_j)
+    double rr = 0.0, ri = 0.0;
+#pragma unroll
+    for (int j = 0; j < MAXL; ++j) {
+      if (j < loops) {
+        int pr = 0, pi = 0;
+#pragma unroll
+        for (int i = 0; i < MAXL; ++i) {
+          if (i < loops) {
+            pr += (vr[i] < vr[j]) || (i < j && vr[i] == vr[j]);
+            pi += (vi[i] < vi[j]) || (i < j && vi[i] == vi[j]);
+          }
+        }
+        if (pr == med) rr = vr[j];
+        if (pi == med) ri = vi[j];
+      }
+    }
+    Node nd;
+    nd.key = I;
+    nd.value.x = rr;
+    nd.value.y = ri;
+    out[h] = nd;
+  }
+}
+
+template <typename Real>
+static hipError_t estimate_dispatch(const int* hits, const int* count, int capacity,
+                                    const void* bins, int loops, int n, const int* ai_dev, int B,
+                                    const double2* win, Node* out, hipStream_t st) {
+  using T2 = typename CT<Real>::T2;
+  const dim3 grid(512), block(256);
+  if (loops <= 8)
+    hipLaunchKernelGGL((estimate_kernel<Real, 8>), grid, block, 0, st, hits, count, capacity,
+                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out);
+  else if (loops <= 16)
+    hipLaunchKernelGGL((estimate_kernel<Real, 16>), grid, block, 0, st, hits, count, capacity,
+                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out);
+  else
+    hipLaunchKernelGGL((estimate_kernel<Real, 32>), grid, block, 0, st, hits, count, capacity,
+                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out);
+  return hipGetLastError();
+}
+
+hipError_t estimate_launch(int is_fp32, const int* hits, const int* count, int capacity,
+                           const void* bins_f, int loops, int n, const int* ai_dev, int B,
+                           const double2* filter_win, Node* out, hipStream_t st) {
+  if (is_fp32)
+    return estimate_dispatch<float>(hits, count, capacity, bins_f, loops, n, ai_dev, B,
+                                    filter_win, out, st);
+  return estimate_dispatch<double>(hits, count, capacity, bins_f, loops, n, ai_dev, B,
+                                   filter_win, out, st);
+}
+
+}  // namespace sfft

@@ -72,18 +72,22 @@ hipError_t perm_filter_launch(int is_fp32, const void* x, const void* f, void* b
 
 // ---- top-K select over `segs` key segments of N keys (stride seg_stride) ----------------
 struct SelectWork {
-  void* state = nullptr;      // per segment: {u64 prefix, int remaining, int pad}
-  unsigned* hist = nullptr;   // segs * 2048
-  unsigned* done = nullptr;   // segs
-  int* cnt = nullptr;         // segs * nchunks * 2
+  void* state = nullptr;      // per segment: threshold, top digit, counters
+  unsigned* hist = nullptr;   // segs * 4096 (zero between calls)
+  unsigned* done = nullptr;   // segs (zero between calls)
+  int* cnt = nullptr;         // segs * nchunks
+  void* cand_key = nullptr;   // segs * N keys
+  int* cand_idx = nullptr;    // segs * N
   int max_segs = 0;
   int max_chunks = 0;
 };
+// N must be a power of two; the workspace must start zeroed
 size_t select_work_bytes(int segs, int N);
 void select_work_bind(SelectWork* w, void* base, int segs, int N);
-// out[s*K + j] = ascending indices of the K largest keys of segment s (ties by lower index)
+// out[s*K + j]: the K largest keys of segment s (ties by lower index); ascending indices when
+// `ordered`, otherwise the same set in arbitrary order
 hipError_t select_topk(int is_fp32, const void* keys, long long seg_stride, int N, int K,
-                       int segs, int* out, const SelectWork& w, hipStream_t st);
+                       int segs, int* out, int ordered, const SelectWork& w, hipStream_t st);
 
 // ---- reverse hash (bitmap of approved residues, votes, deterministic hit list) -----------
 struct HashArgs {
@@ -108,7 +112,7 @@ hipError_t hash_clear_launch(const HashArgs& h, hipStream_t st);
 hipError_t hash_bitmap_launch(const HashArgs& h, hipStream_t st);
 hipError_t hash_vote_launch(const HashArgs& h, hipStream_t st);
 hipError_t hash_hits_launch(const HashArgs& h, hipStream_t st);
-int hash_blocks(int loops_loc, int K);
+int hash_blocks(int loops_loc, int K, long long m);   // m = n / B
 
 // ---- estimation ---------------------------------------------------------------------------
 hipError_t estimate_launch(int is_fp32, const int* hits, const int* count, int capacity,

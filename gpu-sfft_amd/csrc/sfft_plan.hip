@@ -214,6 +214,11 @@ struct sfft_plan {
   const void* d_ft = nullptr;
   int has_filter = 0;
   int last_count = 0;
+  // pinned host staging: draws (a, ai, offsets packed like d_a..d_offs) and the hit count
+  int* h_draws = nullptr;
+  int* h_count = nullptr;
+  hipEvent_t up_ev = nullptr;   // last upload out of h_draws
+  int up_pending = 0;
   // optional stage timing (hipEvents at the reference's stage boundaries)
   int profile = 0;
   hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -285,16 +290,14 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   const size_t W = prm->W_RLFC, B = prm->B;
   const size_t e = P->fp32 ? 8 : 16, ke = P->fp32 ? 4 : 8;
   P->bm_words = (int)((W + 31) / 32);
-  const int nhb = hash_blocks(Ll, K);
+  const int nhb = hash_blocks(Ll, K, (long long)prm->n / prm->B);
   // carve one workspace
   struct Item {
     void** dst;
     size_t bytes;
   };
   std::vector<Item> items;
-  items.push_back({(void**)&P->d_a, (size_t)L * 4});
-  items.push_back({(void**)&P->d_ai, (size_t)L * 4});
-  items.push_back({(void**)&P->d_offs, (size_t)Rl * 4});
+  items.push_back({(void**)&P->d_a, (size_t)(2 * L + Rl) * 4});   // a | ai | offsets
   items.push_back({&P->comb0, (size_t)Rl * W * e});
   items.push_back({&P->comb1, (size_t)Rl * W * e});
   items.push_back({&P->comb_keys, (size_t)Rl * W * ke});
@@ -331,16 +334,19 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
     *it.dst = (char*)P->ws + off;
     off += align_up(it.bytes, 256);
   }
+  P->d_ai = P->d_a + L;
+  P->d_offs = P->d_a + 2 * L;
   select_work_bind(&P->selW, selW_base, Rl, (int)W);
   select_work_bind(&P->selB, selB_base, Ll, (int)B);
   // select state must start at zero (hist/done are reset by the kernels themselves)
   if (hipMemset(P->ws, 0, total) != hipSuccess ||
       fft_tables_create(P->log2B, 0, P->fp32, &P->tabB) != hipSuccess ||
-      fft_tables_create(P->log2W, 0, P->fp32, &P->tabW) != hipSuccess) {
-    fft_tables_destroy(&P->tabB);
-    fft_tables_destroy(&P->tabW);
-    (void)hipFree(P->ws);
-    delete P;
+      fft_tables_create(P->log2W, 0, P->fp32, &P->tabW) != hipSuccess ||
+      hipHostMalloc((void**)&P->h_draws, (size_t)(2 * L + Rl) * 4, hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&P->h_count, 16, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&P->up_ev, hipEventDisableTiming) != hipSuccess) {
+    sfft_plan_destroy(P);
     return SFFT_ERR_ALLOC;
   }
   *out = P;
@@ -349,8 +355,13 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
 
 extern "C" void sfft_plan_destroy(sfft_plan* P) {
   if (!P) return;
+  if (P->st) (void)hipStreamSynchronize(P->st);
+  else (void)hipDeviceSynchronize();
   for (auto& e : P->ev)
     if (e) (void)hipEventDestroy(e);
+  if (P->up_ev) (void)hipEventDestroy(P->up_ev);
+  if (P->h_draws) (void)hipHostFree(P->h_draws);
+  if (P->h_count) (void)hipHostFree(P->h_count);
   fft_tables_destroy(&P->tabB);
   fft_tables_destroy(&P->tabW);
   if (P->ws) (void)hipFree(P->ws);
@@ -398,10 +409,15 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   hipStream_t st = P->st;
   const int L = p.loops, Ll = p.loops_loc, Rl = p.RLFC_loops, K = p.B_thresh;
   const long long W = p.W_RLFC, B = p.B;
-  // draws (pageable source: the copy is staged before hipMemcpyAsync returns)
-  HIPCK(hipMemcpyAsync(P->d_a, dr->a, (size_t)L * 4, hipMemcpyHostToDevice, st));
-  HIPCK(hipMemcpyAsync(P->d_ai, dr->ai, (size_t)L * 4, hipMemcpyHostToDevice, st));
-  HIPCK(hipMemcpyAsync(P->d_offs, dr->offsets, (size_t)Rl * 4, hipMemcpyHostToDevice, st));
+  // draws: packed into pinned staging (after the previous upload out of it has completed),
+  // then one asynchronous copy
+  if (P->up_pending) HIPCK(hipEventSynchronize(P->up_ev));
+  memcpy(P->h_draws, dr->a, (size_t)L * 4);
+  memcpy(P->h_draws + L, dr->ai, (size_t)L * 4);
+  memcpy(P->h_draws + 2 * L, dr->offsets, (size_t)Rl * 4);
+  HIPCK(hipMemcpyAsync(P->d_a, P->h_draws, (size_t)(2 * L + Rl) * 4, hipMemcpyHostToDevice, st));
+  HIPCK(hipEventRecord(P->up_ev, st));
+  P->up_pending = 1;
 
   HashArgs h;
   h.n = p.n;
@@ -449,7 +465,8 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   c.comb_offs = P->d_offs;
   c.comb_sigma = (int)(p.n / p.W_RLFC);
   HIPCK(fft_run(c, st));
-  HIPCK(select_topk(P->fp32, P->comb_keys, W, (int)W, K, Rl, P->J2, P->selW, st));
+  // the comb lists only feed the order-free bitmap build: unordered output
+  HIPCK(select_topk(P->fp32, P->comb_keys, W, (int)W, K, Rl, P->J2, 0, P->selW, st));
   HIPCK(hash_bitmap_launch(h, st));
   HIPCK(mark(2));
 
@@ -472,7 +489,8 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   f.store = kStoreBoth;
   if (P->log2B == 0) return SFFT_ERR_INVALID;
   HIPCK(fft_run(f, st));
-  HIPCK(select_topk(P->fp32, P->bkeys, B, (int)B, K, Ll, P->J, P->selB, st));
+  // ascending bucket lists: the hit list is ordered by (loop, bucket, walk)
+  HIPCK(select_topk(P->fp32, P->bkeys, B, (int)B, K, Ll, P->J, 1, P->selB, st));
   HIPCK(mark(4));
 
   // reverse hash: votes, then the hit list in (loop, bucket, walk) order
@@ -521,9 +539,9 @@ extern "C" sfft_status sfft_execute(sfft_plan* P, const void* d_x, const sfft_dr
   if (!P) return SFFT_ERR_INVALID;
   sfft_status s = sfft_execute_async(P, d_x, dr, P->nodes, P->count);
   if (s != SFFT_OK) return s;
-  int cnt = 0;
-  HIPCK(hipMemcpyAsync(&cnt, P->count, 4, hipMemcpyDeviceToHost, P->st));
+  HIPCK(hipMemcpyAsync(P->h_count, P->count, 4, hipMemcpyDeviceToHost, P->st));
   HIPCK(hipStreamSynchronize(P->st));
+  const int cnt = *P->h_count;
   P->last_count = cnt;
   if (h_count) *h_count = cnt;
   const int ncopy = cnt < P->cap ? cnt : P->cap;

@@ -1,0 +1,463 @@
+// sfft_select.hip -- top-K cutoff on the GPU: the K largest |X|^2 of each segment, ties broken
+// by the lower index (the reference's cutoff, innerLoop.cu:551-642), without sorting.
+//
+// Every element gets a unique composite key  c = (key without sign bit) . (N-1-index)  so that
+// "larger c" means "larger |X|^2, or equal and lower index"; the top-K set is {c >= T}.
+//   1. hist0   : histogram of the top 12 bits of c over all N (wave-aggregated LDS atomics);
+//                the last block of a segment picks the digit d0 holding the K-th largest.
+//   2. split   : one pass over N: digit > d0 -> selected (emitted at once in unordered mode),
+//                digit == d0 -> appended to a candidate list (typically ~1% of N).
+//   3. refine  : one workgroup per segment: MSD radix select on the remaining bits of c over
+//                the candidates only, with early exit once a digit bin fills the remainder;
+//                yields T and (unordered mode) emits the selected candidates.
+//   4. ordered mode only: count + write, an order-preserving compaction of {c >= T} that
+//                produces the ascending index list the reverse hash walks in order.
+#include "sfft_internal.h"
+
+namespace sfft {
+
+typedef unsigned __int128 u128;
+
+struct SelState {
+  unsigned long long thr_hi, thr_lo;   // T as two 64-bit halves
+  int d0;                              // top digit holding the K-th largest
+  int need0;                           // K - #(digit > d0)
+  int cand_count;
+  int out_count;
+};
+
+constexpr int SEL_NT = 512;
+constexpr int SEL_DIG = 12;
+constexpr int SEL_BINS = 1 << SEL_DIG;
+
+template <typename K>
+struct KeyBits;
+template <>
+struct KeyBits<unsigned long long> {
+  static constexpr int KB = 63;  // bits below the (always zero) sign bit
+};
+template <>
+struct KeyBits<unsigned int> {
+  static constexpr int KB = 31;
+};
+
+__device__ __forceinline__ int ilog2_dev(int N) { return 31 - __clz(N); }
+
+constexpr int SEL_LDS_CAND = 4096;   // per-block LDS list capacity in the split pass
+
+// elements per block: about 128 blocks per segment for large N, at least 2048
+static int sel_chunk(int N) {
+  int c = 2048;
+  while ((long long)c * 128 < N) c <<= 1;
+  return c;
+}
+
+size_t select_work_bytes(int segs, int N) {
+  const int nch = (N + sel_chunk(N) - 1) / sel_chunk(N);
+  return (size_t)segs * sizeof(SelState) + (size_t)segs * SEL_BINS * 4 + (size_t)segs * 4 +
+         (size_t)segs * nch * 4 + (size_t)segs * N * 8 + (size_t)segs * N * 4 + 1024;
+}
+
+void select_work_bind(SelectWork* w, void* base, int segs, int N) {
+  const int nch = (N + sel_chunk(N) - 1) / sel_chunk(N);
+  char* p = (char*)base;
+  auto take = [&](size_t bytes) {
+    char* r = p;
+    p += (bytes + 255) / 256 * 256;
+    return r;
+  };
+  w->state = take((size_t)segs * sizeof(SelState));
+  w->hist = (unsigned*)take((size_t)segs * SEL_BINS * 4);
+  w->done = (unsigned*)take((size_t)segs * 4);
+  w->cnt = (int*)take((size_t)segs * nch * 4);
+  w->cand_key = take((size_t)segs * N * 8);
+  w->cand_idx = (int*)take((size_t)segs * N * 4);
+  w->max_segs = segs;
+  w->max_chunks = nch;
+}
+
+__device__ __forceinline__ int block_excl_scan_s(int v, int* sh) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[wid] = x;
+  __syncthreads();
+  int off = 0;
+  for (int i = 0; i < wid; ++i) off += sh[i];
+  __syncthreads();
+  return off + x - v;
+}
+
+// increment h[bin] for the active lanes: one atomic for the whole wave when every active lane
+// holds the same bin (flat or tied data), else one LDS atomic per lane (spread data, where
+// per-bin aggregation would loop once per distinct bin)
+__device__ __forceinline__ void wave_hist_add(unsigned* h, unsigned bin, bool active) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long act = __ballot(active);
+  if (!act) return;
+  const int first = __ffsll((long long)act) - 1;
+  const unsigned b0 = __shfl(bin, first, 64);
+  const unsigned long long same = __ballot(active && bin == b0);
+  if (same == act) {
+    if (lane == first) atomicAdd(&h[b0], (unsigned)__popcll(act));
+  } else if (active) {
+    atomicAdd(&h[bin], 1u);
+  }
+}
+
+// wave-aggregated append: returns the slot of this lane (valid only where pred)
+__device__ __forceinline__ int wave_append(int* counter, bool pred) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long m = __ballot(pred);
+  if (!m) return -1;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, leader, 64);
+  return base + __popcll(m & ((1ull << lane) - 1ull));
+}
+
+template <typename K>
+__device__ __forceinline__ unsigned top_digit(K key) {
+  return (unsigned)((key >> (KeyBits<K>::KB - SEL_DIG)) & (K)(SEL_BINS - 1));
+}
+
+template <typename K>
+__device__ __forceinline__ u128 composite(K key, unsigned idx, int N, int IB) {
+  const K km = key & (K)(~((K)1 << KeyBits<K>::KB));
+  return ((u128)km << IB) | (u128)(unsigned)((unsigned)(N - 1) - idx);
+}
+
+// ---- 1. top-digit histogram; last block picks d0 -----------------------------------------
+template <typename K>
+__global__ __launch_bounds__(SEL_NT) void sel_hist0_kernel(const K* __restrict__ keys,
+                                                           long long seg_stride, int N, int chunk,
+                                                           int nchunks, int Ksel,
+                                                           SelState* __restrict__ st,
+                                                           unsigned* __restrict__ ghist,
+                                                           unsigned* __restrict__ done) {
+  __shared__ unsigned h[SEL_BINS];
+  __shared__ int s_last;
+  __shared__ int sh[16];
+  const int tid = threadIdx.x, seg = blockIdx.y;
+  for (int i = tid; i < SEL_BINS; i += SEL_NT) h[i] = 0u;
+  __syncthreads();
+  const K* kp = keys + (long long)seg * seg_stride;
+  const int begin = blockIdx.x * chunk, end = min(N, begin + chunk);
+  constexpr int U = 8;   // independent loads in flight per thread
+  for (int base = begin; base < end; base += SEL_NT * U) {
+    K kk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * SEL_NT + tid;
+      kk[u] = i < end ? kp[i] : (K)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * SEL_NT + tid;
+      wave_hist_add(h, top_digit<K>(kk[u]), i < end);
+    }
+  }
+  __syncthreads();
+  unsigned* gh = ghist + (size_t)seg * SEL_BINS;
+  for (int i = tid; i < SEL_BINS; i += SEL_NT)
+    if (h[i]) atomicAdd(&gh[i], h[i]);
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) s_last = (atomicAdd(&done[seg], 1u) == (unsigned)(nchunks - 1));
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  // bins from the top: thread t owns ranks [8t, 8t+8)
+  unsigned v[8];
+  int S = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int rank = 8 * tid + j;
+    v[j] = __hip_atomic_load(&gh[SEL_BINS - 1 - rank], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    S += (int)v[j];
+  }
+  const int E = block_excl_scan_s(S, sh);
+  if (E < Ksel && Ksel <= E + S) {
+    int cum = E;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (cum + (int)v[j] >= Ksel) {
+        st[seg].d0 = SEL_BINS - 1 - (8 * tid + j);
+        st[seg].need0 = Ksel - cum;
+        break;
+      }
+      cum += (int)v[j];
+    }
+  }
+  for (int i = tid; i < SEL_BINS; i += SEL_NT) gh[i] = 0u;
+  if (tid == 0) {
+    done[seg] = 0u;
+    st[seg].cand_count = 0;
+    st[seg].out_count = 0;
+  }
+}
+
+// ---- 2. split: emit digit > d0 (unordered mode), collect digit == d0 ---------------------
+template <typename K>
+__global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__ keys,
+                                                           long long seg_stride, int N, int chunk,
+                                                           int Ksel, int emit,
+                                                           SelState* __restrict__ st,
+                                                           K* __restrict__ ckey,
+                                                           int* __restrict__ cidx,
+                                                           int* __restrict__ out) {
+  // candidates and emitted indices are gathered in LDS (one LDS atomic per wave per round)
+  // and reserved in global memory with one atomic per block; a block that outgrows the LDS
+  // lists (degenerate inputs) appends the rest directly
+  __shared__ K lk[SEL_LDS_CAND];
+  __shared__ int li[SEL_LDS_CAND];
+  __shared__ int le[SEL_LDS_CAND];
+  __shared__ int s_nc, s_ne, s_gc, s_ge;
+  const int tid = threadIdx.x, seg = blockIdx.y;
+  if (tid == 0) {
+    s_nc = 0;
+    s_ne = 0;
+  }
+  __syncthreads();
+  const unsigned d0 = (unsigned)st[seg].d0;
+  const K* kp = keys + (long long)seg * seg_stride;
+  K* ck = ckey + (long long)seg * N;
+  int* ci = cidx + (long long)seg * N;
+  int* o = out + (long long)seg * Ksel;
+  const int begin = blockIdx.x * chunk, end = min(N, begin + chunk);
+  for (int base = begin; base < end; base += SEL_NT) {
+    const int i = base + tid;
+    const bool ok = i < end;
+    const K k = ok ? kp[i] : (K)0;
+    const unsigned d = top_digit<K>(k);
+    const bool isc = ok && d == d0;
+    const int pc = wave_append(&s_nc, isc);
+    if (isc) {
+      if (pc < SEL_LDS_CAND) {
+        lk[pc] = k;
+        li[pc] = i;
+      } else {
+        const int g = atomicAdd(&st[seg].cand_count, 1);
+        ck[g] = k;
+        ci[g] = i;
+      }
+    }
+    if (emit) {
+      const bool ise = ok && d > d0;
+      const int pe = wave_append(&s_ne, ise);
+      if (ise) {
+        if (pe < SEL_LDS_CAND) {
+          le[pe] = i;
+        } else {
+          const int g = atomicAdd(&st[seg].out_count, 1);
+          if (g < Ksel) o[g] = i;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int nc = min(s_nc, SEL_LDS_CAND), ne = min(s_ne, SEL_LDS_CAND);
+  if (tid == 0) {
+    s_gc = nc ? atomicAdd(&st[seg].cand_count, nc) : 0;
+    s_ge = ne ? atomicAdd(&st[seg].out_count, ne) : 0;
+  }
+  __syncthreads();
+  for (int j = tid; j < nc; j += SEL_NT) {
+    ck[s_gc + j] = lk[j];
+    ci[s_gc + j] = li[j];
+  }
+  for (int j = tid; j < ne; j += SEL_NT)
+    if (s_ge + j < Ksel) o[s_ge + j] = le[j];
+}
+
+// ---- 3. refine over the candidates (one workgroup per segment) ----------------------------
+template <typename K>
+__global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int emit,
+                                                          SelState* __restrict__ st,
+                                                          const K* __restrict__ ckey,
+                                                          const int* __restrict__ cidx,
+                                                          int* __restrict__ out) {
+  __shared__ unsigned h[SEL_BINS];
+  __shared__ int sh[16];
+  __shared__ int s_digit, s_above, s_exit;
+  const int tid = threadIdx.x, seg = blockIdx.x;
+  const int IB = ilog2_dev(N);
+  const int TB = KeyBits<K>::KB + IB;
+  const int nc = st[seg].cand_count;
+  const K* ck = ckey + (long long)seg * N;
+  const int* ci = cidx + (long long)seg * N;
+  int r = st[seg].need0;
+  u128 prefix = (u128)(unsigned)st[seg].d0;   // the top SEL_DIG bits of c
+  int consumed = SEL_DIG;
+  u128 T = 0;
+  bool done = false;
+  // at most ceil(TB / SEL_DIG) levels; the bound only guards against a broken invariant
+  for (int level = 0; !done && level < (TB + SEL_DIG - 1) / SEL_DIG; ++level) {
+    const int width = min(SEL_DIG, TB - consumed);
+    const int shift = TB - consumed - width;
+    for (int i = tid; i < SEL_BINS; i += 1024) h[i] = 0u;
+    if (tid == 0) {
+      s_digit = 0;
+      s_above = 0;
+      s_exit = 1;
+    }
+    __syncthreads();
+    for (int j = tid; j < nc; j += 1024) {
+      const u128 c = composite<K>(ck[j], (unsigned)ci[j], N, IB);
+      if ((c >> (shift + width)) == prefix)
+        atomicAdd(&h[(unsigned)(c >> shift) & ((1u << width) - 1u)], 1u);
+    }
+    __syncthreads();
+    const int nb = 1 << width;
+    unsigned v[4];
+    int S = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int rank = 4 * tid + j;
+      v[j] = rank < nb ? h[nb - 1 - rank] : 0u;
+      S += (int)v[j];
+    }
+    const int E = block_excl_scan_s(S, sh);   // (contains the barrier after the defaults)
+    if (E < r && r <= E + S) {
+      int cum = E;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (cum + (int)v[j] >= r) {
+          s_digit = nb - 1 - (4 * tid + j);
+          s_above = cum;
+          s_exit = ((cum + (int)v[j] == r) || (shift == 0)) ? 1 : 0;
+          break;
+        }
+        cum += (int)v[j];
+      }
+    }
+    __syncthreads();
+    const unsigned d = (unsigned)s_digit;
+    prefix = (prefix << width) | (u128)d;
+    r -= s_above;
+    consumed += width;
+    if (s_exit) {
+      T = prefix << shift;
+      done = true;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    st[seg].thr_hi = (unsigned long long)(T >> 64);
+    st[seg].thr_lo = (unsigned long long)T;
+  }
+  if (emit) {
+    int* o = out + (long long)seg * Ksel;
+    for (int base = 0; base < nc; base += 1024) {
+      const int j = base + tid;
+      const bool sel = j < nc && composite<K>(ck[j], (unsigned)ci[j], N, IB) >= T;
+      const int po = wave_append(&st[seg].out_count, sel);
+      if (sel && po < Ksel) o[po] = ci[j];
+    }
+  }
+}
+
+// ---- 4. ordered compaction of {c >= T} -----------------------------------------------------
+template <typename K>
+__device__ __forceinline__ bool sel_pred(K k, int i, int N, int IB, u128 T) {
+  return composite<K>(k, (unsigned)i, N, IB) >= T;
+}
+
+template <typename K>
+__global__ __launch_bounds__(SEL_NT) void sel_count_kernel(const K* __restrict__ keys,
+                                                           long long seg_stride, int N, int chunk,
+                                                           int nchunks,
+                                                           const SelState* __restrict__ st,
+                                                           int* __restrict__ cnt) {
+  __shared__ int sh[16];
+  const int seg = blockIdx.y, tid = threadIdx.x;
+  const int IB = ilog2_dev(N);
+  const u128 T = ((u128)st[seg].thr_hi << 64) | (u128)st[seg].thr_lo;
+  const K* kp = keys + (long long)seg * seg_stride;
+  const int begin = blockIdx.x * chunk, end = min(N, begin + chunk);
+  int c = 0;
+  for (int i = begin + tid; i < end; i += SEL_NT) c += sel_pred<K>(kp[i], i, N, IB, T);
+  // block sum
+  const int lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if (lane == 0) sh[wid] = c;
+  __syncthreads();
+  if (tid == 0) {
+    int t = 0;
+    for (int w = 0; w < SEL_NT / 64; ++w) t += sh[w];
+    cnt[(long long)seg * nchunks + blockIdx.x] = t;
+  }
+}
+
+template <typename K>
+__global__ __launch_bounds__(SEL_NT) void sel_write_kernel(const K* __restrict__ keys,
+                                                           long long seg_stride, int N, int chunk,
+                                                           int nchunks, int Ksel,
+                                                           const SelState* __restrict__ st,
+                                                           const int* __restrict__ cnt,
+                                                           int* __restrict__ out) {
+  __shared__ int sh[16];
+  __shared__ int s_base, s_tot;
+  const int seg = blockIdx.y, tid = threadIdx.x;
+  const int IB = ilog2_dev(N);
+  const u128 T = ((u128)st[seg].thr_hi << 64) | (u128)st[seg].thr_lo;
+  const K* kp = keys + (long long)seg * seg_stride;
+  if (tid == 0) {
+    int b = 0;
+    for (int c = 0; c < (int)blockIdx.x; ++c) b += cnt[(long long)seg * nchunks + c];
+    s_base = b;
+  }
+  __syncthreads();
+  int base = s_base;
+  int* o = out + (long long)seg * Ksel;
+  const int begin = blockIdx.x * chunk, end = min(N, begin + chunk);
+  for (int b0 = begin; b0 < end; b0 += SEL_NT) {
+    const int i = b0 + tid;
+    const bool p = i < end && sel_pred<K>(kp[i], i, N, IB, T);
+    const int ex = block_excl_scan_s(p ? 1 : 0, sh);
+    if (p && base + ex < Ksel) o[base + ex] = i;
+    if (tid == SEL_NT - 1) s_tot = ex + (p ? 1 : 0);   // total of this round
+    __syncthreads();
+    base += s_tot;
+    __syncthreads();
+  }
+}
+
+template <typename K>
+static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ksel, int segs,
+                              int* out, int ordered, const SelectWork& w, hipStream_t st) {
+  const int chunk = sel_chunk(N);
+  const int nchunks = (N + chunk - 1) / chunk;
+  dim3 grid(nchunks, segs);
+  SelState* S = (SelState*)w.state;
+  hipLaunchKernelGGL((sel_hist0_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
+                     chunk, nchunks, Ksel, S, w.hist, w.done);
+  hipLaunchKernelGGL((sel_split_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
+                     chunk, Ksel, ordered ? 0 : 1, S, (K*)w.cand_key, w.cand_idx, out);
+  hipLaunchKernelGGL((sel_refine_kernel<K>), dim3(segs), dim3(1024), 0, st, N, Ksel,
+                     ordered ? 0 : 1, S, (const K*)w.cand_key, (const int*)w.cand_idx, out);
+  if (ordered) {
+    hipLaunchKernelGGL((sel_count_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
+                       chunk, nchunks, (const SelState*)S, w.cnt);
+    hipLaunchKernelGGL((sel_write_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
+                       chunk, nchunks, Ksel, (const SelState*)S, (const int*)w.cnt, out);
+  }
+  return hipGetLastError();
+}
+
+hipError_t select_topk(int is_fp32, const void* keys, long long seg_stride, int N, int K,
+                       int segs, int* out, int ordered, const SelectWork& w, hipStream_t st) {
+  if (is_fp32)
+    return select_impl<unsigned int>((const unsigned int*)keys, seg_stride, N, K, segs, out,
+                                     ordered, w, st);
+  return select_impl<unsigned long long>((const unsigned long long*)keys, seg_stride, N, K, segs,
+                                         out, ordered, w, st);
+}
+
+}  // namespace sfft

@@ -119,7 +119,11 @@ class Plan:
                                     _stream_handle(stream), C.byref(h)), "sfft_plan_create")
         self._h = h
         self._ft = None
-        self._nodes = np.zeros(self.capacity, dtype=NODE_DTYPE)
+        # pinned host buffer: the node copy at the end of execute() is a direct DMA
+        torch = _torch()
+        self._nodes_buf = torch.empty(self.capacity * NODE_DTYPE.itemsize, dtype=torch.uint8,
+                                      pin_memory=True)
+        self._nodes = self._nodes_buf.numpy().view(NODE_DTYPE)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -177,15 +181,18 @@ class Plan:
                 not x.is_contiguous():
             raise TypeError(f"x must be a contiguous {want} device tensor of n entries")
 
-    def execute(self, x, a, ai, offsets):
-        """Runs one transform and returns (nodes[I_F] numpy structured array, I_F)."""
+    def execute(self, x, a, ai, offsets, copy=True):
+        """Runs one transform and returns (nodes[I_F] numpy structured array, I_F).  With
+        copy=False the nodes are a view of the plan's pinned buffer (valid until the next
+        execute)."""
         self._check_x(x)
         dr = self._draws(a, ai, offsets)
         cnt = C.c_int()
         st = lib.sfft_execute(self._h, _ptr(x), C.byref(dr), _np_ptr(self._nodes),
                               C.byref(cnt))
         _check(st, "sfft_execute")
-        return self._nodes[:cnt.value].copy(), cnt.value
+        nodes = self._nodes[:cnt.value]
+        return (nodes.copy() if copy else nodes), cnt.value
 
     def execute_async(self, x, a, ai, offsets, d_nodes=None, d_count=None):
         """Enqueues one transform on the plan stream; results stay on the device."""

@@ -92,6 +92,10 @@ def _declare(L):
     L.orc_outer_loop64.argtypes = [C.POINTER(Plan), _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                                    _p, _p, _p]
     L.orc_outer_loop64.restype = _i
+    L.orc_outer_loop32.argtypes = [C.POINTER(Plan), _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                   _p, _p]
+    L.orc_outer_loop32.restype = _i
+    L.orc_rlfc32.argtypes = [_p, _i, _i, _i, _i, _p]
     L.orc_metric_dense.argtypes = [_p, _i, _i, _p, _i, _p, _p, _p, _p]
 
 
@@ -319,6 +323,38 @@ def outer_loop64(prm, x, filter_time, filter_win, a, ai, offsets, fold_mode=FOLD
                                  _ptr(J) if J is not None else None, _ptr(J2), C.byref(nj2),
                                  _ptr(st))
     res = {"I_F": I_F, "stage_seconds": st, "J2": J2[:nj2.value].copy()}
+    if I_F >= 0:
+        res["hits"] = hits[:I_F].copy()
+        res["nodes"] = nodes[:I_F].copy()
+    if bins is not None:
+        res["bins_f"] = bins.reshape(prm["loops"], prm["B"])
+    if J is not None:
+        res["J"] = J.reshape(prm["loops_loc"], prm["B_thresh"])
+    return res
+
+
+def outer_loop32(prm, x, filter_time, filter_win, a, ai, offsets, fold_mode=FOLD_REFERENCE,
+                 hit_capacity=524288, want_bins=False, want_J=False):
+    """The FP32 path with explicit draws (x and filter_time complex64)."""
+    pl = Plan(n=prm["n"], B=prm["B"], B_thresh=prm["B_thresh"], W_RLFC=prm["W_RLFC"],
+              RLFC_loops=prm["RLFC_loops"], loops_thresh=prm["loops_thresh"],
+              loops_loc=prm["loops_loc"], loops=prm["loops"], w=int(filter_time.size),
+              fold_mode=fold_mode, hit_capacity=hit_capacity)
+    x = _c64(x)
+    ft = _c64(filter_time)
+    fw = _c128(filter_win)
+    a, ai, offsets = _i32(a), _i32(ai), _i32(offsets)
+    hits = np.zeros(hit_capacity, dtype=np.int32)
+    nodes = np.zeros(hit_capacity, dtype=NODE_DTYPE)
+    bins = np.zeros(prm["loops"] * prm["B"], dtype=np.complex64) if want_bins else None
+    J = np.zeros(prm["loops_loc"] * prm["B_thresh"], dtype=np.int32) if want_J else None
+    J2 = np.zeros(prm["RLFC_loops"] * prm["B_thresh"], dtype=np.int32)
+    nj2 = C.c_int()
+    I_F = lib().orc_outer_loop32(C.byref(pl), _ptr(x), _ptr(ft), _ptr(fw), _ptr(a), _ptr(ai),
+                                 _ptr(offsets), _ptr(hits), _ptr(nodes),
+                                 _ptr(bins) if bins is not None else None,
+                                 _ptr(J) if J is not None else None, _ptr(J2), C.byref(nj2))
+    res = {"I_F": I_F, "J2": J2[:nj2.value].copy()}
     if I_F >= 0:
         res["hits"] = hits[:I_F].copy()
         res["nodes"] = nodes[:I_F].copy()

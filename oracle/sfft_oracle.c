@@ -528,8 +528,24 @@ static void insertion_sort(double *v, int n) {                /* insertionSortDe
   }
 }
 
+/* bins are complex128 (bins_f) or complex64 (bins32, widened exactly to double) */
+static void estimate_values_any(const int *hits, int I_F, const orc_c64 *bins_f,
+                                const orc_c32 *bins32, int loops, int n, const int *ai, int B,
+                                const orc_c64 *filter_win, orc_node *out);
+
 void orc_estimate_values(const int *hits, int I_F, const orc_c64 *bins_f, int loops, int n,
                          const int *ai, int B, const orc_c64 *filter_win, orc_node *out) {
+  estimate_values_any(hits, I_F, bins_f, NULL, loops, n, ai, B, filter_win, out);
+}
+
+void orc_estimate_values32(const int *hits, int I_F, const orc_c32 *bins_f, int loops, int n,
+                           const int *ai, int B, const orc_c64 *filter_win, orc_node *out) {
+  estimate_values_any(hits, I_F, NULL, bins_f, loops, n, ai, B, filter_win, out);
+}
+
+static void estimate_values_any(const int *hits, int I_F, const orc_c64 *bins_f,
+                                const orc_c32 *bins32, int loops, int n, const int *ai, int B,
+                                const orc_c64 *filter_win, orc_node *out) {
   const int m = n / B;
   const long long nmask = (long long)n - 1;
   double *vr = (double *)malloc(sizeof(double) * (size_t)loops);
@@ -547,7 +563,13 @@ void orc_estimate_values(const int *hits, int I_F, const orc_c64 *bins_f, int lo
       dist = ((long long)n - dist) & nmask;
       const long long widx = (dist + m / 2) & nmask;
       const orc_c64 F = filter_win[widx];
-      const orc_c64 b = bins_f[(size_t)l * B + hashed];
+      orc_c64 b;
+      if (bins_f) {
+        b = bins_f[(size_t)l * B + hashed];
+      } else {
+        b.x = (double)bins32[(size_t)l * B + hashed].x;
+        b.y = (double)bins32[(size_t)l * B + hashed].y;
+      }
       const double abs2 = fma(F.x, F.x, F.y * F.y);
       const double mr = fma(b.x, F.x, b.y * F.y);
       const double mi = fma(b.y, F.x, -(b.x * F.y));
@@ -633,6 +655,57 @@ int orc_outer_loop64(const orc_plan *p, const orc_c64 *x, const orc_c64 *filter_
     st[0] = T_vote; st[1] = T_rlfc; st[2] = T_pf; st[3] = T_rh; st[4] = T_est; st[5] = T_fc;
     st[6] = now_s() - t0;
   }
+  return I_F;
+}
+
+/* FP32 path: complex64 signal and taps, FP32 comb and bucket FFTs and |X|^2 keys, Kahan
+ * folding; the filter division and medians in FP64 on the widened bins. */
+void orc_rlfc32(const orc_c32 *x, int n, int W, int offset, int B_thresh, int *J2_out) {
+  const int sigma = n / W;
+  orc_c32 *y = (orc_c32 *)malloc(sizeof(orc_c32) * (size_t)W);
+  orc_c32 *scr = (orc_c32 *)malloc(sizeof(orc_c32) * (size_t)W);
+  float *s = (float *)malloc(sizeof(float) * (size_t)W);
+  for (int i = 0; i < W; ++i) y[i] = x[(size_t)offset + (size_t)i * sigma];
+  orc_fft32(y, ilog2_exact((uint64_t)W), 0, scr);
+  orc_samples32(y, W, s);
+  orc_cutoff32(s, W, B_thresh, J2_out);
+  free(y); free(scr); free(s);
+}
+
+int orc_outer_loop32(const orc_plan *p, const orc_c32 *x, const orc_c32 *filter_time,
+                     const orc_c64 *filter_win, const int *a, const int *ai, const int *offsets,
+                     int *hits_out, orc_node *nodes_out, orc_c32 *bins_f_out, int *J_out,
+                     int *J2_out, int *num_RLFC_out) {
+  const int n = p->n, B = p->B, K = p->B_thresh, W = p->W_RLFC;
+  int *vote = (int *)calloc((size_t)n, sizeof(int));
+  int *J2 = (int *)malloc(sizeof(int) * (size_t)p->RLFC_loops * K);
+  for (int r = 0; r < p->RLFC_loops; ++r) orc_rlfc32(x, n, W, offsets[r], K, J2 + (size_t)r * K);
+  int num_RLFC = K;
+  if (p->RLFC_loops > 1) num_RLFC = orc_merge_J2(J2, p->RLFC_loops * K);
+  orc_c32 *bins_f = (orc_c32 *)malloc(sizeof(orc_c32) * (size_t)p->loops * B);
+  orc_c32 *scr = (orc_c32 *)malloc(sizeof(orc_c32) * (size_t)B);
+  float *s = (float *)malloc(sizeof(float) * (size_t)B);
+  int *J = (int *)malloc(sizeof(int) * (size_t)K);
+  int I_F = 0;
+  const int lB = ilog2_exact((uint64_t)B);
+  for (int l = 0; l < p->loops && I_F >= 0; ++l) {
+    orc_c32 *row = bins_f + (size_t)l * B;
+    orc_perm_filter32(x, n, filter_time, p->w, B, ai[l], p->fold_mode, row);
+    orc_fft32(row, lB, 0, scr);
+    if (l < p->loops_loc) {
+      orc_samples32(row, B, s);
+      orc_cutoff32(s, B, K, J);
+      if (J_out) memcpy(J_out + (size_t)l * K, J, sizeof(int) * (size_t)K);
+      I_F = orc_reverse_hash(J, K, n, B, a[l], ai[l], p->loops_thresh, vote, hits_out, I_F,
+                             p->hit_capacity, J2, num_RLFC, W);
+    }
+  }
+  if (I_F >= 0)
+    orc_estimate_values32(hits_out, I_F, bins_f, p->loops, n, ai, B, filter_win, nodes_out);
+  if (bins_f_out) memcpy(bins_f_out, bins_f, sizeof(orc_c32) * (size_t)p->loops * B);
+  if (J2_out) memcpy(J2_out, J2, sizeof(int) * (size_t)num_RLFC);
+  if (num_RLFC_out) *num_RLFC_out = num_RLFC;
+  free(vote); free(J2); free(bins_f); free(scr); free(s); free(J);
   return I_F;
 }
 

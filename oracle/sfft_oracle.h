@@ -141,6 +141,18 @@ int orc_outer_loop64(const orc_plan *p, const orc_c64 *x, const orc_c64 *filter_
                      int *hits_out, orc_node *nodes_out, orc_c64 *bins_f_out, int *J_out,
                      int *J2_out, int *num_RLFC_out, double *stage_seconds);
 
+/* FP32 path (the project's mixed-precision addition, not in the reference): complex64 x and
+ * taps, FP32 comb/bucket FFTs and keys, Kahan-compensated fold, FP64 filter division and
+ * medians on the widened bins.  pinned by: the FP32 stage pins above plus error bounds against
+ * the FP64 path in test_oracle_fp32. */
+void orc_rlfc32(const orc_c32 *x, int n, int W, int offset, int B_thresh, int *J2_out);
+void orc_estimate_values32(const int *hits, int I_F, const orc_c32 *bins_f, int loops, int n,
+                           const int *ai, int B, const orc_c64 *filter_win, orc_node *out);
+int orc_outer_loop32(const orc_plan *p, const orc_c32 *x, const orc_c32 *filter_time,
+                     const orc_c64 *filter_win, const int *a, const int *ai, const int *offsets,
+                     int *hits_out, orc_node *nodes_out, orc_c32 *bins_f_out, int *J_out,
+                     int *J2_out, int *num_RLFC_out);
+
 /* ---- harness metric (run_experiment, cuExperiment.cu:181-221) ---------------------------
  * qsort by |value| descending (comp_struct, utils.c:231-235), keep the first k in a dense
  * n-vector, MISSED = (k - FOUND, k - large_found) with FOUND by key search over all I_F

@@ -115,6 +115,45 @@ def test_transform_cfg1_bitexact(S):
     assert set(locs.tolist()) <= set(nodes["key"].tolist())
 
 
+@pytest.mark.parametrize("kind", ["constant", "one_tone", "integer"])
+def test_transform_degenerate_signals_ties(S, kind):
+    """Signals whose comb and bucket spectra are mostly exact zeros: the cutoffs must pick the
+    lowest indices among the tied keys exactly like the oracle."""
+    prm = params(SMALL_CONFIGS[1])
+    n, B = prm["n"], prm["B"]
+    rng = np.random.default_rng(17)
+    if kind == "constant":
+        x = np.ones(n, dtype=np.complex128)
+    elif kind == "one_tone":
+        x = np.exp(2j * np.pi * 123 * np.arange(n) / n)
+    else:
+        x = (rng.integers(-2, 3, n) + 1j * rng.integers(-2, 3, n)).astype(np.complex128)
+    ft, ff = oracle_filter(prm)
+    win = O.filter_window_slice(ff, n, B)
+    a, ai, offs = draws(n, prm["loops"], prm["RLFC_loops"], prm["W_RLFC"], 21)
+    ref = O.outer_loop64(prm, x, ft, win, a, ai, offs, want_bins=True, want_J=True)
+    plan = S.Plan(prm, "fp64")
+    plan.set_filter(_dev(ft), win)
+    nodes, cnt = plan.execute(_dev(x), a, ai, offs)
+    _assert_transform_equal(prm, plan, ref, nodes, cnt)
+
+
+@pytest.mark.parametrize("N,K", [(1 << 10, 100), (1 << 14, 4000), (1 << 17, 32000)])
+def test_samples_indices_with_ties(S, N, K):
+    """Reference entry point samples_indices (innerLoop.cu:484-539) on keys with ties."""
+    import ctypes as C
+    rng = np.random.default_rng(N)
+    X = (rng.integers(-3, 4, N) + 1j * rng.integers(-3, 4, N)).astype(np.complex128)
+    dX = _dev(X)
+    import torch
+    out = torch.zeros(K, dtype=torch.int32, device="cuda")
+    t = C.c_double(0.0)
+    assert S.lib.samples_indices(C.c_void_p(dX.data_ptr()), K, N, C.c_void_p(out.data_ptr()),
+                                 C.byref(t)) == 0
+    want = O.cutoff64(O.samples64(X), K)
+    assert np.array_equal(out.cpu().numpy(), want)
+
+
 def test_transform_deterministic(S):
     """two runs of the same plan give identical bits (no racy folds, ordered hit list)."""
     prm = params(SMALL_CONFIGS[1])
@@ -163,3 +202,73 @@ def test_dft_any_matches_numpy(S):
         assert np.array_equal(bits(got), bits(O.dft_any64(x)))
         ref = np.fft.fft(x)
         assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_reference_outer_loop_entry_point(S):
+    """outer_loop (outerLoop.h:35-38) draws with libc drand48()/random() in the reference's
+    order; replaying those draws through the oracle gives the same nodes."""
+    import ctypes as C
+    libc = C.CDLL("libc.so.6")
+    libc.drand48.restype = C.c_double
+    libc.random.restype = C.c_long
+    prm = params(SMALL_CONFIGS[1])
+    n, B = prm["n"], prm["B"]
+    x, locs, _ = sparse_signal(n, prm["k"], 77)
+    ft, ff = oracle_filter(prm)
+    # the draws outer_loop will make
+    libc.srand48(C.c_long(4321))
+    libc.srandom(C.c_uint(8765))
+    sigma = n // prm["W_RLFC"]
+    offs = [int(np.floor(libc.drand48() * sigma)) for _ in range(prm["RLFC_loops"])]
+    a = []
+    for _ in range(prm["loops"]):
+        v = 0
+        while np.gcd(v, n) != 1:
+            v = int(libc.random() % n)
+        a.append(v)
+    ai = [pow(v, -1, n) for v in a]
+    libc.srand48(C.c_long(4321))
+    libc.srandom(C.c_uint(8765))
+    dx, dft = _dev(x), _dev(ft)
+    I_F = C.c_int(0)
+    C.c_int.in_dll(S.lib, "TIMING").value = 0
+    p = S.lib.outer_loop(C.c_void_p(dx.data_ptr()), n, C.c_void_p(dft.data_ptr()),
+                         ff.ctypes.data_as(C.c_void_p), prm["w"], B, prm["B_thresh"],
+                         prm["W_RLFC"], prm["RLFC_loops"], prm["loops_thresh"], prm["loops_loc"],
+                         prm["loops"], C.byref(I_F), C.c_float(0.0))
+    C.c_int.in_dll(S.lib, "TIMING").value = 1
+    assert p
+    got = np.frombuffer(C.string_at(p, I_F.value * 24), dtype=O.NODE_DTYPE).copy()
+    C.CDLL("libc.so.6").free(C.c_void_p(C.cast(p, C.c_void_p).value))
+    win = O.filter_window_slice(ff, n, B)
+    ref = O.outer_loop64(prm, x, ft, win, np.array(a, np.int32), np.array(ai, np.int32),
+                         np.array(offs, np.int32))
+    assert I_F.value == ref["I_F"]
+    assert got.tobytes() == ref["nodes"].tobytes()
+    assert set(locs.tolist()) <= set(got["key"].tolist())
+
+
+# ---- the FP32 path: complex64 storage, FP32 FFTs and Kahan fold, FP64 estimation ---------
+@pytest.mark.parametrize("cfg", [SMALL_CONFIGS[1], SMALL_CONFIGS[3], CFG1])
+def test_transform_fp32_bitexact(S, cfg):
+    prm = params(cfg)
+    n, B, L, Ll, Rl, K = (prm[k] for k in ("n", "B", "loops", "loops_loc", "RLFC_loops",
+                                            "B_thresh"))
+    x, locs, _ = sparse_signal(n, prm["k"], 99)
+    x32 = x.astype(np.complex64)
+    ft, ff = oracle_filter(prm)
+    ft32 = ft.astype(np.complex64)
+    win = O.filter_window_slice(ff, n, B)
+    a, ai, offs = draws(n, L, Rl, prm["W_RLFC"], 101)
+    ref = O.outer_loop32(prm, x32, ft32, win, a, ai, offs, want_bins=True, want_J=True)
+    plan = S.Plan(prm, "fp32")
+    plan.set_filter(_dev(ft32), win)
+    nodes, cnt = plan.execute(_dev(x32), a, ai, offs)
+    J2 = plan.debug_array(2, Rl * K, np.int32)
+    assert np.array_equal(np.unique(J2), ref["J2"])
+    bins_f = plan.debug_array(0, L * B, np.complex64).reshape(L, B)
+    assert np.array_equal(bits(bins_f), bits(ref["bins_f"]))
+    assert np.array_equal(plan.debug_array(1, Ll * K, np.int32).reshape(Ll, K), ref["J"])
+    assert cnt == ref["I_F"]
+    assert np.array_equal(plan.debug_array(3, cnt, np.int32), ref["hits"])
+    assert nodes.tobytes() == ref["nodes"].tobytes()
