@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--front-end", default="auto", choices=["auto", "gather", "stream"],
+                    help="signal access: gathers per tap (auto), or one streaming pass")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget for the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=0x5FF70002)
@@ -79,7 +81,8 @@ def main():
     log(f"[bench] filter design (FP64, w={w}): {time.time() - t:.2f}s")
     a, ai, offs = S.draw_permutations(n, L, prm["RLFC_loops"], prm["W_RLFC"], args.seed + 2)
 
-    plan = S.Plan(prm, args.precision)
+    front = "gather" if args.front_end == "auto" else args.front_end
+    plan = S.Plan(prm, args.precision, front_end=front)
     plan.set_filter(ft, win)
     plan.set_profiling(True)
 
@@ -88,6 +91,7 @@ def main():
         nodes, cnt = plan.execute(x, a, ai, offs, copy=False)
     torch.cuda.synchronize()
     stage_sum = None
+    iv_sum = None
     step_ms = []
     t_all = time.perf_counter()
     for _ in range(args.steps):
@@ -96,10 +100,13 @@ def main():
         step_ms.append((time.perf_counter() - t0) * 1e3)
         st = plan.stage_times()
         stage_sum = st if stage_sum is None else {kk: stage_sum[kk] + v for kk, v in st.items()}
+        iv = plan.interval_times()
+        iv_sum = iv if iv_sum is None else {kk: iv_sum[kk] + v for kk, v in iv.items()}
     torch.cuda.synchronize()
     total_s = time.perf_counter() - t_all
     ms_per_step = total_s * 1e3 / args.steps
     stages_ms = {kk: v * 1e3 / args.steps for kk, v in stage_sum.items()}
+    iv_ms = {kk: v * 1e3 / args.steps for kk, v in iv_sum.items()}
     log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (median "
         f"{float(np.median(step_ms)):.3f}); stages(ms) "
         + " ".join(f"{kk}={v:.3f}" for kk, v in stages_ms.items()))
@@ -108,20 +115,27 @@ def main():
     log(f"[bench] I_F={cnt} missed={acc['missed_est']},{acc['missed_res']} "
         f"MSE*k={acc['mse_k']:.4e} MSE={acc['mse']:.4e}")
 
-    # ---- roofline of the dominant kernel: permute+filter ----
+    # ---- roofline of the dominant kernel ----
+    # gather front end: permute+filter, L*w random 64-byte granules + taps + bins.
+    # streaming front end: the streaming pass, reads x once, writes the L*w tap rows and the
+    # Rl*W comb samples.
     e = 8 if fp32 else 16
-    pf_bytes = L * w * GATHER_GRANULE + w * e + L * B * e
-    pf_s = stages_ms["perm+filter"] * 1e-3
-    achieved = pf_bytes / pf_s / 1e9
+    Wc, Rl = prm["W_RLFC"], prm["RLFC_loops"]
+    if front == "stream":
+        kname, k_bytes, k_ms = ("stream_scatter_kernel", n * e + L * w * e + Rl * Wc * e,
+                                iv_ms["stream"])
+    else:
+        kname, k_bytes, k_ms = ("perm_filter_kernel", L * w * GATHER_GRANULE + w * e + L * B * e,
+                                iv_ms["fold"])
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get(f"perm_filter_{args.config}_{args.precision}")
+            traffic = json.load(fh).get(f"{kname}_{args.config}_{args.precision}")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "perm_filter_kernel", "bytes_per_launch": pf_bytes,
-                "launch_ms": round(stages_ms["perm+filter"], 4)}
+                "kernel": kname, "bytes_per_launch": k_bytes, "launch_ms": round(k_ms, 4)}
     # whole-transform algorithmic rate (SURVEY formula: gathers + taps + nodes)
     sig = n // prm["W_RLFC"]
     g_c = GATHER_GRANULE if sig * e >= GATHER_GRANULE else sig * e
@@ -157,15 +171,16 @@ def main():
         "higher_is_better": False, "scaling": "weak",
         "vs_baseline": round(ms_per_step / PUBLISHED_MS, 6) if args.config == "cfg2" and
         not fp32 else None,
-        "dtype": "f32" if fp32 else "f64", "data": "synthetic",
+        "dtype": "f32" if fp32 else "f64", "data": "synthetic", "front_end": front,
         "config": {"workload": f"{args.config}: sfft n=2^{q} k={k} Bcst={Bcst} "
                                f"loops={ll}+{le} thresh={lt} RLFC={rc}x{rl} "
                                f"{'noise ' + str(snr) + 'dB' if snr else 'noiseless'}, "
-                               f"{args.precision} path",
+                               f"{args.precision} path, {front} front end",
                    "n": n, "k": k, "B": B, "W_RLFC": prm["W_RLFC"], "w": w, "loops": L,
                    "parallelism": "none (1 GPU)"},
         "median_ms": round(float(np.median(step_ms)), 4),
         "stages_ms": {kk: round(v, 4) for kk, v in stages_ms.items()},
+        "intervals_ms": {kk: round(v, 4) for kk, v in iv_ms.items()},
         "roofline": roofline, "transform_roofline": transform,
         "accuracy": {"I_F": cnt, "missed": [acc["missed_est"], acc["missed_res"]],
                      "mse_k": acc["mse_k"], "mse": acc["mse"], "max_abs": acc["max_abs"]},

@@ -95,6 +95,7 @@ extern "C" Node* outer_loop(complex_t* d_x, int n, complex_t* d_filter_time,
   p.w = d_filter_size;
   p.fold_mode = SFFT_FOLD_REFERENCE;
   p.hit_capacity = MAXSIZE_HITS;
+  p.front_end = SFFT_FRONT_AUTO;
   // draws in the reference's order: one drand48 per comb loop (innerLoop.cu:122-123), then
   // random() until gcd(a, n) == 1 per loop (outerLoop.cu:129-134)
   std::vector<int> a(loops), ai(loops), offs(RLFC_loops);

@@ -417,7 +417,11 @@ struct AccOf<float> {
 
 constexpr int PF_NT = 256;
 
-template <typename Real, int LPT>
+// bins[l*B + r] = sum over the taps id of bucket r (canonical order) of v_l(id) * f[id], where
+// v_l(id) = x[(id * ai_l) mod n] gathered from the signal (TAPS = false), or read from the
+// tap-ordered staging array T[l*w + id] that the streaming pass filled (TAPS = true; same
+// values, coalesced reads).  Each thread owns one bucket of LPT loops.
+template <typename Real, int LPT, bool TAPS>
 __global__ __launch_bounds__(PF_NT) void perm_filter_kernel(
     const typename CT<Real>::T2* __restrict__ x, const typename CT<Real>::T2* __restrict__ f,
     typename CT<Real>::T2* __restrict__ bins, const int* __restrict__ ai_dev, uint32_t mask,
@@ -428,17 +432,19 @@ __global__ __launch_bounds__(PF_NT) void perm_filter_kernel(
   if (r >= B) return;
   const int l0 = blockIdx.y * LPT;
   uint32_t ai[LPT];
+  const T2* __restrict__ src[LPT];
   AC acc[LPT];
 #pragma unroll
   for (int li = 0; li < LPT; ++li) {
     ai[li] = (l0 + li < loops) ? (uint32_t)ai_dev[l0 + li] : 0u;
+    src[li] = x + (size_t)min(l0 + li, loops - 1) * (size_t)w;   // staging rows (TAPS)
     acc[li].init();
   }
   auto tap = [&](uint32_t id) {
     const T2 fv = f[id];
     T2 xv[LPT];
 #pragma unroll
-    for (int li = 0; li < LPT; ++li) xv[li] = x[(id * ai[li]) & mask];
+    for (int li = 0; li < LPT; ++li) xv[li] = TAPS ? src[li][id] : x[(id * ai[li]) & mask];
 #pragma unroll
     for (int li = 0; li < LPT; ++li) acc[li].add(xv[li], fv);
   };
@@ -456,20 +462,93 @@ __global__ __launch_bounds__(PF_NT) void perm_filter_kernel(
     if (l0 + li < loops) bins[(long long)(l0 + li) * B + r] = acc[li].get();
 }
 
-hipError_t perm_filter_launch(int is_fp32, const void* x, const void* f, void* bins,
-                              const int* ai_dev, int n, int B, int w, int loops, int tiled,
-                              hipStream_t st) {
+template <bool TAPS>
+static hipError_t pf_launch(int is_fp32, const void* src, const void* f, void* bins,
+                            const int* ai_dev, int n, int B, int w, int loops, int tiled,
+                            hipStream_t st) {
   constexpr int LPT = 3;
   dim3 grid((B + PF_NT - 1) / PF_NT, (loops + LPT - 1) / LPT);
   const uint32_t mask = (uint32_t)n - 1u;
   if (is_fp32)
-    hipLaunchKernelGGL((perm_filter_kernel<float, LPT>), grid, dim3(PF_NT), 0, st,
-                       (const float2*)x, (const float2*)f, (float2*)bins, ai_dev, mask, B, w,
+    hipLaunchKernelGGL((perm_filter_kernel<float, LPT, TAPS>), grid, dim3(PF_NT), 0, st,
+                       (const float2*)src, (const float2*)f, (float2*)bins, ai_dev, mask, B, w,
                        loops, tiled);
   else
-    hipLaunchKernelGGL((perm_filter_kernel<double, LPT>), grid, dim3(PF_NT), 0, st,
-                       (const double2*)x, (const double2*)f, (double2*)bins, ai_dev, mask, B, w,
-                       loops, tiled);
+    hipLaunchKernelGGL((perm_filter_kernel<double, LPT, TAPS>), grid, dim3(PF_NT), 0, st,
+                       (const double2*)src, (const double2*)f, (double2*)bins, ai_dev, mask, B,
+                       w, loops, tiled);
+  return hipGetLastError();
+}
+
+hipError_t perm_filter_launch(int is_fp32, const void* x, const void* f, void* bins,
+                              const int* ai_dev, int n, int B, int w, int loops, int tiled,
+                              hipStream_t st) {
+  return pf_launch<false>(is_fp32, x, f, bins, ai_dev, n, B, w, loops, tiled, st);
+}
+
+hipError_t fold_taps_launch(int is_fp32, const void* T, const void* f, void* bins,
+                            const int* ai_dev, int n, int B, int w, int loops, int tiled,
+                            hipStream_t st) {
+  return pf_launch<true>(is_fp32, T, f, bins, ai_dev, n, B, w, loops, tiled, st);
+}
+
+// ============================================================================================
+// streaming pass: read the signal once in address order and route every element to the tap
+// slots and comb samples that use it:
+//   T[l*w + i] = x[m]  for each loop l with i = (m * a_l) mod n < w   (i.e. m = i * ai_l)
+//   y[r*W + j] = x[m]  for each comb r with m = off_r + j * sigma
+// Each thread handles SS_EPT consecutive elements; i advances by a_l per element (no multiply).
+// ============================================================================================
+constexpr int SS_NT = 256;
+constexpr int SS_EPT = 4;
+
+template <typename T2>
+__global__ __launch_bounds__(SS_NT) void stream_scatter_kernel(
+    const T2* __restrict__ x, T2* __restrict__ T, T2* __restrict__ y,
+    const int* __restrict__ a_dev, const int* __restrict__ offs, uint32_t n, uint32_t w,
+    int loops, int Rl, int log2sigma, uint32_t W) {
+  const uint32_t mask = n - 1u;
+  const uint32_t m0 = (blockIdx.x * SS_NT + threadIdx.x) * SS_EPT;
+  if (m0 >= n) return;
+  T2 v[SS_EPT];
+#pragma unroll
+  for (int e = 0; e < SS_EPT; ++e) v[e] = x[m0 + e];
+  for (int l = 0; l < loops; ++l) {
+    const uint32_t a = (uint32_t)a_dev[l];
+    uint32_t i = m0 * a;
+    T2* __restrict__ Tl = T + (size_t)l * w;
+#pragma unroll
+    for (int e = 0; e < SS_EPT; ++e) {
+      const uint32_t ii = i & mask;
+      if (ii < w) Tl[ii] = v[e];
+      i += a;
+    }
+  }
+  const uint32_t smask = (1u << log2sigma) - 1u;
+  for (int r = 0; r < Rl; ++r) {
+    const uint32_t d0 = m0 - (uint32_t)offs[r];   // wraps for m < off: never a multiple
+#pragma unroll
+    for (int e = 0; e < SS_EPT; ++e) {
+      const uint32_t d = d0 + (uint32_t)e;
+      if ((d & smask) == 0u) y[(size_t)r * W + (d >> log2sigma)] = v[e];
+    }
+  }
+}
+
+hipError_t stream_scatter_launch(int is_fp32, const void* x, void* T, void* y, const int* a_dev,
+                                 const int* offs, int n, int w, int loops, int Rl, int W,
+                                 hipStream_t st) {
+  int log2sigma = 0;
+  while ((W << log2sigma) < n) ++log2sigma;
+  const unsigned blocks = (unsigned)(((long long)n / SS_EPT + SS_NT - 1) / SS_NT);
+  if (is_fp32)
+    hipLaunchKernelGGL((stream_scatter_kernel<float2>), dim3(blocks), dim3(SS_NT), 0, st,
+                       (const float2*)x, (float2*)T, (float2*)y, a_dev, offs, (uint32_t)n,
+                       (uint32_t)w, loops, Rl, log2sigma, (uint32_t)W);
+  else
+    hipLaunchKernelGGL((stream_scatter_kernel<double2>), dim3(blocks), dim3(SS_NT), 0, st,
+                       (const double2*)x, (double2*)T, (double2*)y, a_dev, offs, (uint32_t)n,
+                       (uint32_t)w, loops, Rl, log2sigma, (uint32_t)W);
   return hipGetLastError();
 }
 

@@ -69,6 +69,15 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st);
 hipError_t perm_filter_launch(int is_fp32, const void* x, const void* f, void* bins,
                               const int* ai_dev, int n, int B, int w, int loops, int tiled,
                               hipStream_t st);
+// same fold, reading the tap values from the staging array T[l*w + i] of the streaming pass
+hipError_t fold_taps_launch(int is_fp32, const void* T, const void* f, void* bins,
+                            const int* ai_dev, int n, int B, int w, int loops, int tiled,
+                            hipStream_t st);
+// streaming pass over x: T[l*w + i] = x[m] for i = m * a_l mod n < w, and the comb samples
+// y[r*W + j] = x[off_r + j * n/W]
+hipError_t stream_scatter_launch(int is_fp32, const void* x, void* T, void* y, const int* a_dev,
+                                 const int* offs, int n, int w, int loops, int Rl, int W,
+                                 hipStream_t st);
 
 // ---- top-K select over `segs` key segments of N keys (stride seg_stride) ----------------
 struct SelectWork {

@@ -193,6 +193,9 @@ struct sfft_plan {
   void* comb0 = nullptr;
   void* comb1 = nullptr;
   void* comb_keys = nullptr;
+  int stream = 0;               // streaming front end (else gathers)
+  void* taps = nullptr;         // loops x w tap values (stream)
+  void* comb_y = nullptr;       // RLFC_loops x W comb samples (stream)
   void* bins_t = nullptr;
   void* bins_scr = nullptr;
   void* bins_f = nullptr;
@@ -221,7 +224,8 @@ struct sfft_plan {
   int up_pending = 0;
   // optional stage timing (hipEvents at the reference's stage boundaries)
   int profile = 0;
-  hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[9] = {nullptr, nullptr, nullptr, nullptr, nullptr,
+                      nullptr, nullptr, nullptr, nullptr};
 };
 
 static int ilog2_exact(long long v) {
@@ -261,6 +265,7 @@ extern "C" sfft_status sfft_params_validate(const sfft_params* p) {
   if (p->fold_mode != SFFT_FOLD_REFERENCE && p->fold_mode != SFFT_FOLD_ALL_TAPS)
     return SFFT_ERR_INVALID;
   if (p->hit_capacity < 0) return SFFT_ERR_INVALID;
+  if (p->front_end < SFFT_FRONT_AUTO || p->front_end > SFFT_FRONT_STREAM) return SFFT_ERR_INVALID;
   if (p->W_RLFC < p->n / p->B) return SFFT_ERR_INVALID;   // the comb block must span a window
   return SFFT_OK;
 }
@@ -298,6 +303,11 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   };
   std::vector<Item> items;
   items.push_back({(void**)&P->d_a, (size_t)(2 * L + Rl) * 4});   // a | ai | offsets
+  P->stream = prm->front_end == SFFT_FRONT_STREAM;   // AUTO = GATHER
+  if (P->stream) {
+    items.push_back({&P->taps, (size_t)L * prm->w * e});
+    items.push_back({&P->comb_y, (size_t)Rl * W * e});
+  }
   items.push_back({&P->comb0, (size_t)Rl * W * e});
   items.push_back({&P->comb1, (size_t)Rl * W * e});
   items.push_back({&P->comb_keys, (size_t)Rl * W * ke});
@@ -449,7 +459,14 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   HIPCK(hash_clear_launch(h, st));
   HIPCK(mark(1));
 
-  // RLFC: comb gather + W-point FFT, keys of |Y|^2 out of the last pass
+  // streaming front end: one pass over x fills the tap staging array and the comb samples
+  if (P->stream)
+    HIPCK(stream_scatter_launch(P->fp32, d_x, P->taps, P->comb_y, P->d_a, P->d_offs, p.n, p.w,
+                                L, Rl, p.W_RLFC, st));
+  HIPCK(mark(2));
+
+  // RLFC: comb samples (gathered in pass 0, or from the streaming pass) + W-point FFT, keys
+  // of |Y|^2 out of the last pass
   FftLaunch c;
   c.tab = &P->tabW;
   c.out = P->comb0;
@@ -459,22 +476,32 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   c.keys_bstride = W;
   c.key_batches = Rl;
   c.batch = Rl;
-  c.load = kLoadComb;
   c.store = kStoreKeys;
-  c.comb_x = d_x;
-  c.comb_offs = P->d_offs;
-  c.comb_sigma = (int)(p.n / p.W_RLFC);
+  if (P->stream) {
+    c.in = P->comb_y;
+    c.in_bstride = W;
+    c.load = kLoadPlain;
+  } else {
+    c.load = kLoadComb;
+    c.comb_x = d_x;
+    c.comb_offs = P->d_offs;
+    c.comb_sigma = (int)(p.n / p.W_RLFC);
+  }
   HIPCK(fft_run(c, st));
   // the comb lists only feed the order-free bitmap build: unordered output
   HIPCK(select_topk(P->fp32, P->comb_keys, W, (int)W, K, Rl, P->J2, 0, P->selW, st));
   HIPCK(hash_bitmap_launch(h, st));
-  HIPCK(mark(2));
+  HIPCK(mark(3));
 
   // permutation + filtering of every loop, then the batched bucket FFT
   const int tiled = (p.fold_mode == SFFT_FOLD_REFERENCE) && (p.n < (1 << 27));
-  HIPCK(perm_filter_launch(P->fp32, d_x, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, L, tiled,
-                           st));
-  HIPCK(mark(3));
+  if (P->stream)
+    HIPCK(fold_taps_launch(P->fp32, P->taps, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, L,
+                           tiled, st));
+  else
+    HIPCK(perm_filter_launch(P->fp32, d_x, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, L,
+                             tiled, st));
+  HIPCK(mark(4));
   FftLaunch f;
   f.tab = &P->tabB;
   f.in = P->bins_t;
@@ -487,21 +514,20 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   f.key_batches = Ll;
   f.batch = L;
   f.store = kStoreBoth;
-  if (P->log2B == 0) return SFFT_ERR_INVALID;
   HIPCK(fft_run(f, st));
   // ascending bucket lists: the hit list is ordered by (loop, bucket, walk)
   HIPCK(select_topk(P->fp32, P->bkeys, B, (int)B, K, Ll, P->J, 1, P->selB, st));
-  HIPCK(mark(4));
+  HIPCK(mark(5));
 
   // reverse hash: votes, then the hit list in (loop, bucket, walk) order
   HIPCK(hash_vote_launch(h, st));
   HIPCK(hash_hits_launch(h, st));
-  HIPCK(mark(5));
+  HIPCK(mark(6));
 
   // estimation
   HIPCK(estimate_launch(P->fp32, P->hits, h.count, P->cap, P->bins_f, L, p.n, P->d_ai, p.B,
                         P->win, d_nodes ? d_nodes : P->nodes, st));
-  HIPCK(mark(6));
+  HIPCK(mark(7));
   return SFFT_OK;
 }
 
@@ -514,22 +540,33 @@ extern "C" sfft_status sfft_plan_set_profiling(sfft_plan* P, int on) {
   return SFFT_OK;
 }
 
-// Seconds per stage of the last profiled transform, in the reference's table order
-// (outerLoop.cu:174-187): votetable, restr-loc-index, perm+filter, reverse-hash, estimation,
-// FFT+Cutoff, total.  Call after the transform completed.
-extern "C" sfft_status sfft_plan_stage_times(sfft_plan* P, double* out7) {
+// Seconds between the plan's stage events of the last profiled transform: clear, streaming
+// pass (0 with gathers), comb FFT + cutoff + bitmaps, permute+filter (or tap fold), bucket
+// FFT + cutoff, reverse hash, estimation.
+extern "C" sfft_status sfft_plan_interval_times(sfft_plan* P, double* out7) {
   if (!P || !out7 || !P->profile) return SFFT_ERR_INVALID;
-  float t[6];
-  for (int i = 0; i < 6; ++i)
-    if (hipEventElapsedTime(&t[i], P->ev[i], P->ev[i + 1]) != hipSuccess) return SFFT_ERR_HIP;
-  out7[0] = t[0] * 1e-3;  // votetable: vote/bitmap clear
-  out7[1] = t[1] * 1e-3;  // restr-loc-index: comb FFT + cutoff + bitmaps
-  out7[2] = t[2] * 1e-3;  // perm+filter
-  out7[5] = t[3] * 1e-3;  // FFT+Cutoff: bucket FFT + location cutoffs
-  out7[3] = t[4] * 1e-3;  // reverse-hash
-  out7[4] = t[5] * 1e-3;  // estimation
+  for (int i = 0; i < 7; ++i) {
+    float t = 0;
+    if (hipEventElapsedTime(&t, P->ev[i], P->ev[i + 1]) != hipSuccess) return SFFT_ERR_HIP;
+    out7[i] = t * 1e-3;
+  }
+  return SFFT_OK;
+}
+
+// Seconds per stage in the reference's table order (outerLoop.cu:174-187): votetable,
+// restr-loc-index, perm+filter, reverse-hash, estimation, FFT+Cutoff, total.
+extern "C" sfft_status sfft_plan_stage_times(sfft_plan* P, double* out7) {
+  double t[7];
+  const sfft_status s = sfft_plan_interval_times(P, t);
+  if (s != SFFT_OK) return s;
+  out7[0] = t[0];            // votetable: vote/bitmap clear
+  out7[1] = t[2];            // restr-loc-index: comb FFT + cutoff + bitmaps
+  out7[2] = t[1] + t[3];     // perm+filter (streaming pass + fold, or gathers)
+  out7[3] = t[5];            // reverse-hash
+  out7[4] = t[6];            // estimation
+  out7[5] = t[4];            // FFT+Cutoff: bucket FFT + location cutoffs
   float tot = 0;
-  if (hipEventElapsedTime(&tot, P->ev[0], P->ev[6]) != hipSuccess) return SFFT_ERR_HIP;
+  if (hipEventElapsedTime(&tot, P->ev[0], P->ev[7]) != hipSuccess) return SFFT_ERR_HIP;
   out7[6] = tot * 1e-3;
   return SFFT_OK;
 }

@@ -28,7 +28,7 @@ class Params(C.Structure):
     _fields_ = [("n", C.c_int), ("B", C.c_int), ("B_thresh", C.c_int), ("W_RLFC", C.c_int),
                 ("RLFC_loops", C.c_int), ("loops_thresh", C.c_int), ("loops_loc", C.c_int),
                 ("loops", C.c_int), ("w", C.c_int), ("fold_mode", C.c_int),
-                ("hit_capacity", C.c_int)]
+                ("hit_capacity", C.c_int), ("front_end", C.c_int)]
 
 
 class Draws(C.Structure):
@@ -48,6 +48,9 @@ SFFT_FP64 = 0
 SFFT_FP32 = 1
 SFFT_FOLD_REFERENCE = 0
 SFFT_FOLD_ALL_TAPS = 1
+SFFT_FRONT_AUTO = 0
+SFFT_FRONT_GATHER = 1
+SFFT_FRONT_STREAM = 2
 
 # every symbol include/sfft.h declares (functions and data)
 EXPORTED_FUNCTIONS = [
@@ -56,7 +59,8 @@ EXPORTED_FUNCTIONS = [
     "run_experiment", "cuShift",
     "sfft_status_string", "sfft_params_validate", "sfft_plan_create", "sfft_plan_destroy",
     "sfft_plan_workspace_bytes", "sfft_plan_set_filter", "sfft_execute_async", "sfft_execute",
-    "sfft_plan_set_profiling", "sfft_plan_stage_times", "sfft_plan_debug_buffer",
+    "sfft_plan_set_profiling", "sfft_plan_stage_times", "sfft_plan_interval_times",
+    "sfft_plan_debug_buffer",
     "sfft_fft_device", "sfft_twiddle", "sfft_fft_plan_passes", "sfft_version",
     "sfft_device_count", "sfft_filter_taps", "sfft_make_filter", "sfft_filter_window",
     "sfft_dft_any_device", "sfft_convert_c128_to_c64", "sfft_convert_c64_to_c128",
@@ -104,6 +108,8 @@ def _declare(L):
     L.sfft_plan_set_profiling.restype = _i
     L.sfft_plan_stage_times.argtypes = [_p, _p]
     L.sfft_plan_stage_times.restype = _i
+    L.sfft_plan_interval_times.argtypes = [_p, _p]
+    L.sfft_plan_interval_times.restype = _i
     L.sfft_plan_debug_buffer.argtypes = [_p, _i]
     L.sfft_plan_debug_buffer.restype = _p
     L.sfft_fft_device.argtypes = [_p, _p, _i, _i, _i, _i, _p]

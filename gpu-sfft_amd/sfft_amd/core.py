@@ -18,6 +18,7 @@ assert NODE_DTYPE.itemsize == C.sizeof(L.Node) == 24
 
 FOLD_REFERENCE = L.SFFT_FOLD_REFERENCE
 FOLD_ALL_TAPS = L.SFFT_FOLD_ALL_TAPS
+FRONT = {"auto": L.SFFT_FRONT_AUTO, "gather": L.SFFT_FRONT_GATHER, "stream": L.SFFT_FRONT_STREAM}
 
 
 class SfftError(RuntimeError):
@@ -89,17 +90,18 @@ def draw_permutations(n, loops, RLFC_loops, W, seed):
     return a.astype(np.int32), ai.astype(np.int32), offs.astype(np.int32)
 
 
-def validate(params, fold_mode=FOLD_REFERENCE, hit_capacity=0):
-    p = _cparams(params, fold_mode, hit_capacity)
+def validate(params, fold_mode=FOLD_REFERENCE, hit_capacity=0, front_end="auto"):
+    p = _cparams(params, fold_mode, hit_capacity, front_end)
     return lib.sfft_params_validate(C.byref(p))
 
 
-def _cparams(params, fold_mode, hit_capacity):
+def _cparams(params, fold_mode, hit_capacity, front_end="auto"):
+    fe = FRONT[front_end] if isinstance(front_end, str) else int(front_end)
     return L.Params(n=params["n"], B=params["B"], B_thresh=params["B_thresh"],
                     W_RLFC=params["W_RLFC"], RLFC_loops=params["RLFC_loops"],
                     loops_thresh=params["loops_thresh"], loops_loc=params["loops_loc"],
                     loops=params["loops"], w=params["w"], fold_mode=fold_mode,
-                    hit_capacity=hit_capacity)
+                    hit_capacity=hit_capacity, front_end=fe)
 
 
 # ---- plan ------------------------------------------------------------------------------------
@@ -107,12 +109,12 @@ class Plan:
     """One transform configuration on the current device (outer_loop with explicit draws)."""
 
     def __init__(self, params, precision="fp64", fold_mode=FOLD_REFERENCE, hit_capacity=0,
-                 stream=None):
+                 stream=None, front_end="auto"):
         self.params = dict(params)
         self.precision = precision
         self.fp32 = precision == "fp32"
         self.stream = stream
-        self._cp = _cparams(params, fold_mode, hit_capacity)
+        self._cp = _cparams(params, fold_mode, hit_capacity, front_end)
         self.capacity = hit_capacity if hit_capacity > 0 else 524288
         h = C.c_void_p()
         _check(lib.sfft_plan_create(C.byref(self._cp), L.SFFT_FP32 if self.fp32 else L.SFFT_FP64,
@@ -165,6 +167,12 @@ class Plan:
         _check(lib.sfft_plan_stage_times(self._h, _np_ptr(out)), "sfft_plan_stage_times")
         return dict(zip(["votetable", "restr-loc-index", "perm+filter", "reverse-hash",
                          "estimation", "FFT+Cutoff", "total"], out.tolist()))
+
+    def interval_times(self):
+        out = np.zeros(7, dtype=np.float64)
+        _check(lib.sfft_plan_interval_times(self._h, _np_ptr(out)), "sfft_plan_interval_times")
+        return dict(zip(["clear", "stream", "comb", "fold", "bucket_fft_cutoff", "hash",
+                         "estimate"], out.tolist()))
 
     def _draws(self, a, ai, offsets):
         self._a = np.ascontiguousarray(a, dtype=np.int32)

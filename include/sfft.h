@@ -135,7 +135,17 @@ typedef struct {
   int w;              /* filter taps */
   int fold_mode;      /* sfft_fold_mode */
   int hit_capacity;   /* 0 -> MAXSIZE_HITS */
+  int front_end;      /* sfft_front_end: how the taps and comb samples are read from x */
 } sfft_params;
+
+/* Both front ends produce bit-identical results.  GATHER reads x[(i*ai) mod n] per tap and
+ * x[off + j*sigma] per comb sample (random HBM reads).  STREAM reads x once in address order
+ * and scatters each used element to a tap-ordered staging array (loops*w entries) and the
+ * comb arrays, which the fold then reads contiguously.  The scatter is as random as the
+ * gathers it replaces (one partial cache line per used element), so on MI355X at n = 2^27 it
+ * is slower (FP64) or even (FP32); AUTO therefore picks GATHER.  STREAM stays available for
+ * signals whose used fraction loops*w/n approaches one. */
+typedef enum { SFFT_FRONT_AUTO = 0, SFFT_FRONT_GATHER = 1, SFFT_FRONT_STREAM = 2 } sfft_front_end;
 
 typedef struct sfft_plan sfft_plan;
 
@@ -178,6 +188,9 @@ sfft_status sfft_execute(sfft_plan *plan, const void *d_x, const sfft_draws *dra
  * estimation, FFT+Cutoff, total. */
 sfft_status sfft_plan_set_profiling(sfft_plan *plan, int on);
 sfft_status sfft_plan_stage_times(sfft_plan *plan, double *seconds7);
+/* raw intervals: clear, streaming pass (0 with gathers), comb FFT + cutoff + bitmaps,
+ * permute+filter (or the tap fold), bucket FFT + cutoff, reverse hash, estimation */
+sfft_status sfft_plan_interval_times(sfft_plan *plan, double *seconds7);
 
 /* Debug/parity taps into the last transform (device pointers owned by the plan):
  * 0 bins_f (loops x B complex), 1 J (loops_loc x B_thresh int), 2 J2 lists (RLFC_loops x

@@ -44,7 +44,7 @@ def test_pod_layouts(S):
     from sfft_amd import _lib
     assert C.sizeof(_lib.complex_t) == 16
     assert C.sizeof(_lib.Node) == 24 and _lib.Node.value.offset == 8
-    assert C.sizeof(_lib.Params) == 11 * 4
+    assert C.sizeof(_lib.Params) == 12 * 4
     assert S.NODE_DTYPE.itemsize == 24
 
 
@@ -90,6 +90,7 @@ def test_validation_mirrors_reference_asserts(S):
         assert S.validate(q) == 1, q
     assert S.validate(p, fold_mode=7) == 1
     assert S.validate(p, hit_capacity=-1) == 1
+    assert S.validate(p, front_end=3) == 1 and S.validate(p, front_end="stream") == 0
 
 
 def test_plan_without_device_reports_status(S):
@@ -106,7 +107,7 @@ def _plan_raw(S, prm):
     p = _lib.Params(n=prm["n"], B=prm["B"], B_thresh=prm["B_thresh"], W_RLFC=prm["W_RLFC"],
                     RLFC_loops=prm["RLFC_loops"], loops_thresh=prm["loops_thresh"],
                     loops_loc=prm["loops_loc"], loops=prm["loops"], w=prm["w"], fold_mode=0,
-                    hit_capacity=0)
+                    hit_capacity=0, front_end=0)
     h = C.c_void_p()
     st = S.lib.sfft_plan_create(C.byref(p), 0, None, C.byref(h))
     if st != 0:

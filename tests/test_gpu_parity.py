@@ -53,7 +53,8 @@ def test_fft32_bitexact(S, log2n):
 
 
 # ---- the whole transform, FP64 plain path -----------------------------------------------------
-def _run_both(S, cfg, seed, fold_mode=O.FOLD_REFERENCE, snr_db=None, prm_override=None):
+def _run_both(S, cfg, seed, fold_mode=O.FOLD_REFERENCE, snr_db=None, prm_override=None,
+              front_end="auto"):
     prm = params(cfg)
     if prm_override:
         prm.update(prm_override)
@@ -64,7 +65,7 @@ def _run_both(S, cfg, seed, fold_mode=O.FOLD_REFERENCE, snr_db=None, prm_overrid
     a, ai, offs = draws(n, prm["loops"], prm["RLFC_loops"], prm["W_RLFC"], seed + 7)
     ref = O.outer_loop64(prm, x, ft, win, a, ai, offs, fold_mode=fold_mode, want_bins=True,
                          want_J=True)
-    plan = S.Plan(prm, "fp64", fold_mode=fold_mode)
+    plan = S.Plan(prm, "fp64", fold_mode=fold_mode, front_end=front_end)
     plan.set_filter(_dev(ft), win)
     nodes, cnt = plan.execute(_dev(x), a, ai, offs)
     return prm, plan, ref, nodes, cnt, locs
@@ -100,6 +101,18 @@ def test_transform_small_bitexact(S, cfg):
 @pytest.mark.parametrize("cfg", SMALL_CONFIGS[:2])
 def test_transform_all_taps_bitexact(S, cfg):
     prm, plan, ref, nodes, cnt, _ = _run_both(S, cfg, seed=5, fold_mode=O.FOLD_ALL_TAPS)
+    _assert_transform_equal(prm, plan, ref, nodes, cnt)
+
+
+@pytest.mark.parametrize("cfg", SMALL_CONFIGS + [CFG1])
+@pytest.mark.parametrize("fold_mode", [O.FOLD_REFERENCE, O.FOLD_ALL_TAPS])
+def test_transform_stream_front_end_bitexact(S, cfg, fold_mode):
+    """Streaming front end (one pass over x fills the tap rows and the comb samples, then the
+    fold reads the rows): same arithmetic as the gather path, so the same bits."""
+    if fold_mode == O.FOLD_ALL_TAPS and cfg == CFG1:
+        pytest.skip("covered by the small configurations")
+    prm, plan, ref, nodes, cnt, _ = _run_both(S, cfg, seed=cfg[0] + 3, fold_mode=fold_mode,
+                                              front_end="stream")
     _assert_transform_equal(prm, plan, ref, nodes, cnt)
 
 
@@ -250,7 +263,8 @@ def test_reference_outer_loop_entry_point(S):
 
 # ---- the FP32 path: complex64 storage, FP32 FFTs and Kahan fold, FP64 estimation ---------
 @pytest.mark.parametrize("cfg", [SMALL_CONFIGS[1], SMALL_CONFIGS[3], CFG1])
-def test_transform_fp32_bitexact(S, cfg):
+@pytest.mark.parametrize("front_end", ["gather", "stream"])
+def test_transform_fp32_bitexact(S, cfg, front_end):
     prm = params(cfg)
     n, B, L, Ll, Rl, K = (prm[k] for k in ("n", "B", "loops", "loops_loc", "RLFC_loops",
                                             "B_thresh"))
@@ -261,7 +275,7 @@ def test_transform_fp32_bitexact(S, cfg):
     win = O.filter_window_slice(ff, n, B)
     a, ai, offs = draws(n, L, Rl, prm["W_RLFC"], 101)
     ref = O.outer_loop32(prm, x32, ft32, win, a, ai, offs, want_bins=True, want_J=True)
-    plan = S.Plan(prm, "fp32")
+    plan = S.Plan(prm, "fp32", front_end=front_end)
     plan.set_filter(_dev(ft32), win)
     nodes, cnt = plan.execute(_dev(x32), a, ai, offs)
     J2 = plan.debug_array(2, Rl * K, np.int32)
