@@ -21,7 +21,10 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-sfft_amd"))
 METRIC = ("ms per SFFT at n=2^27,k=16000 (FP32 vs FP64, MSE≤1e-4); % HBM peak")
 PUBLISHED_MS = 172.697          # README.md:102 (Titan Xp, same parameters, FP64)
 HBM_PEAK_GBS = 8000.0
-GATHER_GRANULE = 64             # bytes per random gather (HBM access granule)
+# bytes one random 8- or 16-byte gather moves from HBM: MI355X issues every read request to
+# memory as one 128-byte line (a 1 GiB copy makes 2^23 requests, and the permute+filter
+# kernel makes one request per tap it gathers).
+GATHER_GRANULE = 128
 
 CONFIGS = {
     # name: (q, k, Bcst, loops_loc, loops_thresh, loops_est, RLFC_cst, RLFC_loops, snr_db)
@@ -43,6 +46,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget for the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=0x5FF70002)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB before the transforms "
+                         "(a known-size kernel that calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -85,6 +91,16 @@ def main():
     plan = S.Plan(prm, args.precision, front_end=front)
     plan.set_filter(ft, win)
     plan.set_profiling(True)
+
+    if args.copy_probe_mib > 0:
+        nb = args.copy_probe_mib << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {nb} bytes read, {nb} bytes written")
+        del src, dst
 
     # ---- warm-up, then K timed transforms (Node[I_F] host-visible at the end of each) ----
     for _ in range(args.warmup):
@@ -135,7 +151,12 @@ def main():
             traffic = json.load(fh).get(f"{kname}_{args.config}_{args.precision}")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": kname, "bytes_per_launch": k_bytes, "launch_ms": round(k_ms, 4)}
+                "kernel": kname, "bytes_per_launch": k_bytes, "launch_ms": round(k_ms, 4),
+                "granule_bytes": GATHER_GRANULE if front == "gather" else None}
+    if front == "gather":
+        # the same kernel priced with a 64-byte granule (half a line per gather)
+        b64 = L * w * 64 + w * e + L * B * e
+        roofline["achieved_granule64"] = round(b64 / (k_ms * 1e-3) / 1e9, 1)
     # whole-transform algorithmic rate (SURVEY formula: gathers + taps + nodes)
     sig = n // prm["W_RLFC"]
     g_c = GATHER_GRANULE if sig * e >= GATHER_GRANULE else sig * e

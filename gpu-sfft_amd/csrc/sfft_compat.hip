@@ -1,6 +1,7 @@
 // sfft_compat.hip -- the reference's C entry points (innerLoop.h, outerLoop.h, filters.h,
 // experiment.h) on top of the MI355X kernels, with the reference's argument meaning,
 // printed output and libc random draws.  The native plan API does the work.
+#include <stdlib.h>
 #include <sys/time.h>
 
 #include <algorithm>
@@ -8,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "sfft_aux.h"
@@ -38,6 +40,39 @@ static double wall_now() {
   struct timeval t;
   gettimeofday(&t, NULL);
   return (double)t.tv_sec + 1e-6 * (double)t.tv_usec;
+}
+
+// The permutation draws of outer_loop (outerLoop.cu:129-134) use glibc's random() generator
+// (the TYPE_3 additive-feedback generator with a 128-byte state), but on a library-private
+// state: the HIP runtime calls random() from its own threads, so once a device is
+// initialised the process-wide state advances by a timing-dependent amount and the draws
+// would differ from run to run.  The private state starts at seed 1, which is the state of
+// a program that never calls srandom(); sfft_srandom reseeds it.  drand48() (comb offsets,
+// signal locations) is not touched by the runtime and stays the process-wide generator.
+static std::mutex g_rand_mu;
+static struct random_data g_rand;
+static char g_rand_state[128];
+static bool g_rand_ready = false;
+
+static void rand_ready_locked() {
+  if (g_rand_ready) return;
+  memset(&g_rand, 0, sizeof g_rand);
+  initstate_r(1u, g_rand_state, sizeof g_rand_state, &g_rand);
+  g_rand_ready = true;
+}
+
+extern "C" void sfft_srandom(unsigned int seed) {
+  std::lock_guard<std::mutex> lk(g_rand_mu);
+  rand_ready_locked();
+  srandom_r(seed, &g_rand);
+}
+
+extern "C" long sfft_random(void) {
+  std::lock_guard<std::mutex> lk(g_rand_mu);
+  rand_ready_locked();
+  int32_t r = 0;
+  random_r(&g_rand, &r);
+  return (long)r;
 }
 
 static int gcd_(int a, int b) {   // utils.c:81-84
@@ -103,7 +138,7 @@ extern "C" Node* outer_loop(complex_t* d_x, int n, complex_t* d_filter_time,
   for (int r = 0; r < RLFC_loops; ++r) offs[r] = (int)(unsigned)floor(drand48() * sigma);
   for (int l = 0; l < loops; ++l) {
     int av = 0;
-    while (gcd_(av, n) != 1) av = (int)(random() % n);
+    while (gcd_(av, n) != 1) av = (int)(sfft_random() % n);
     a[l] = av;
     ai[l] = mod_inverse_(av, n);
   }

@@ -56,7 +56,7 @@ SFFT_FRONT_STREAM = 2
 EXPORTED_FUNCTIONS = [
     "outer_loop", "restr_loc_freq_coeff", "samples_indices", "perm_filter_cutoff",
     "reverse_hash", "estimate_values", "timesmod", "make_multiple_t", "cudafft_experiment",
-    "run_experiment", "cuShift",
+    "run_experiment", "cuShift", "sfft_srandom", "sfft_random",
     "sfft_status_string", "sfft_params_validate", "sfft_plan_create", "sfft_plan_destroy",
     "sfft_plan_workspace_bytes", "sfft_plan_set_filter", "sfft_execute_async", "sfft_execute",
     "sfft_plan_set_profiling", "sfft_plan_stage_times", "sfft_plan_interval_times",
@@ -142,6 +142,10 @@ def _declare(L):
                               C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), _p]
     L.sfft_metric.restype = _i
     # reference entry points
+    L.sfft_srandom.argtypes = [C.c_uint]
+    L.sfft_srandom.restype = None
+    L.sfft_random.argtypes = []
+    L.sfft_random.restype = C.c_long
     L.timesmod.argtypes = [_i, _i, _i]
     L.timesmod.restype = _i
     L.outer_loop.argtypes = [_p, _i, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, C.POINTER(_i),

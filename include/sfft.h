@@ -53,8 +53,9 @@ extern bool_t TIMING;
 /* outerLoop.h:35-38.  d_x: device complex_t[n].  d_filter_time: device complex_t[w].
  * d_filter_freq: complex_t[n], host-readable or device (only the m = n/B entries around 0
  * are read).  Returns a malloc'd host Node[*I_F] (caller frees), NULL on failure.
- * Permutations and RLFC offsets are drawn with libc random()/drand48() in the reference's
- * order (outerLoop.cu:128-134, innerLoop.cu:122-123). */
+ * Permutations and RLFC offsets are drawn with glibc's random() generator (library-private
+ * state, see sfft_srandom) and libc drand48() in the reference's order (outerLoop.cu:128-134,
+ * innerLoop.cu:122-123). */
 Node *outer_loop(complex_t *d_x, int n, complex_t *d_filter_time, complex_t *d_filter_freq,
                  int d_filter_size, int B, int B_thresh, int W_RLFC, int RLFC_loops,
                  int loop_threshold, int loops_loc, int loops, int *I_F, float t_H_to_D);
@@ -87,6 +88,15 @@ Node *estimate_values(int *d_I, int *I_F, complex_t *d_bins_f, int loops, int n,
 
 /* innerLoop.h:50 */
 int timesmod(const int x, const int a, const int n);
+
+/* outer_loop draws its permutations a[l] with glibc's random() generator in the reference's
+ * order (outerLoop.cu:129-134) -- on a library-private state, because the HIP runtime calls
+ * random() from its own threads and would make the process-wide sequence timing-dependent.
+ * The private state starts as a program that never called srandom() (seed 1), so the
+ * reference's command line gets the reference's draws; these reseed it and draw from it
+ * (the counterparts of srandom()/random() for callers that set or replay the draws). */
+void sfft_srandom(unsigned int seed);
+long sfft_random(void);
 
 /* filters.h:44-48.  Designs the filter (FP64) for d_x's size and runs the experiment
  * (run_experiment) exactly as the reference's harness does. */

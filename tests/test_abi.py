@@ -141,6 +141,39 @@ def test_filter_taps_match(S):
         assert S.filter_taps(p["lobefrac"], p["tolerance"]) == p["w"]
 
 
+class _RandomData(C.Structure):   # glibc struct random_data
+    _fields_ = [("fptr", C.c_void_p), ("rptr", C.c_void_p), ("state", C.c_void_p),
+                ("rand_type", C.c_int), ("rand_deg", C.c_int), ("rand_sep", C.c_int),
+                ("end_ptr", C.c_void_p)]
+
+
+def _glibc_random(seed, count):
+    """count draws of glibc's random() generator from srandom(seed), on a fresh private
+    state (random_r), as a process that never touched the global state would draw them"""
+    libc = C.CDLL("libc.so.6")
+    buf = C.create_string_buffer(128)
+    rd = _RandomData()
+    assert libc.initstate_r(C.c_uint(seed), buf, C.c_size_t(128), C.byref(rd)) == 0
+    out, r = [], C.c_int32()
+    for _ in range(count):
+        libc.random_r(C.byref(rd), C.byref(r))
+        out.append(r.value)
+    return out
+
+
+def test_permutation_generator_is_glibc_random(S):
+    """outer_loop's permutation draws come from glibc's random() generator on a private state
+    that starts at seed 1 (a program that never calls srandom()); 1804289383 is the first
+    value such a program's random() returns."""
+    S.lib.sfft_srandom(1)
+    got = [S.lib.sfft_random() for _ in range(50)]
+    assert got[0] == 1804289383
+    assert got == _glibc_random(1, 50)
+    S.lib.sfft_srandom(8765)
+    assert [S.lib.sfft_random() for _ in range(50)] == _glibc_random(8765, 50)
+    S.lib.sfft_srandom(1)
+
+
 def test_status_strings(S):
     for s in range(7):
         assert S.lib.sfft_status_string(s)

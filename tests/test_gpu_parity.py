@@ -218,30 +218,30 @@ def test_dft_any_matches_numpy(S):
 
 
 def test_reference_outer_loop_entry_point(S):
-    """outer_loop (outerLoop.h:35-38) draws with libc drand48()/random() in the reference's
-    order; replaying those draws through the oracle gives the same nodes."""
+    """outer_loop (outerLoop.h:35-38) draws with drand48() and glibc's random() generator
+    (library-private state) in the reference's order; replaying those draws through the
+    oracle gives the same nodes."""
     import ctypes as C
     libc = C.CDLL("libc.so.6")
     libc.drand48.restype = C.c_double
-    libc.random.restype = C.c_long
     prm = params(SMALL_CONFIGS[1])
     n, B = prm["n"], prm["B"]
     x, locs, _ = sparse_signal(n, prm["k"], 77)
     ft, ff = oracle_filter(prm)
     # the draws outer_loop will make
     libc.srand48(C.c_long(4321))
-    libc.srandom(C.c_uint(8765))
+    S.lib.sfft_srandom(8765)
     sigma = n // prm["W_RLFC"]
     offs = [int(np.floor(libc.drand48() * sigma)) for _ in range(prm["RLFC_loops"])]
     a = []
     for _ in range(prm["loops"]):
         v = 0
         while np.gcd(v, n) != 1:
-            v = int(libc.random() % n)
+            v = int(S.lib.sfft_random() % n)
         a.append(v)
     ai = [pow(v, -1, n) for v in a]
     libc.srand48(C.c_long(4321))
-    libc.srandom(C.c_uint(8765))
+    S.lib.sfft_srandom(8765)
     dx, dft = _dev(x), _dev(ft)
     I_F = C.c_int(0)
     C.c_int.in_dll(S.lib, "TIMING").value = 0
