@@ -382,7 +382,8 @@ extern "C" Node* estimate_values(int* d_I, int* I_F, complex_t* d_bins_f, int lo
             hipMemcpy(dperm, permute, (size_t)loops * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(dcnt, &cnt, 4, hipMemcpyHostToDevice) == hipSuccess &&
             estimate_launch(0, d_I, dcnt, cnt, d_bins_f, loops, n, dperm, B,
-                            (const double2*)dwin, (Node*)dnodes, nullptr) == hipSuccess &&
+                            (const double2*)dwin, (Node*)dnodes, nullptr, nullptr) ==
+                hipSuccess &&
             hipMemcpy(h, dnodes, sizeof(Node) * (size_t)cnt, hipMemcpyDeviceToHost) == hipSuccess;
   if (dwin) (void)hipFree(dwin);
   if (dnodes) (void)hipFree(dnodes);

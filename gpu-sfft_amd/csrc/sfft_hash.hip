@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void estimate_kernel(
     const int* __restrict__ hits, const int* __restrict__ count, int capacity,
     const typename CT<Real>::T2* __restrict__ bins, int loops, uint32_t n,
     const int* __restrict__ ai_dev, int B, const double2* __restrict__ win,
-    Node* __restrict__ out) {
+    Node* __restrict__ out, uint8_t* __restrict__ sel) {
   const int total = min(*count, capacity);
   const long long m = (long long)(n / (uint32_t)B);
   const long long nn = n;
@@ -326,6 +326,7 @@ __global__ __launch_bounds__(256) void estimate_kernel(
     }
     // stable rank selection: position of j in an insertion sort = #(v_i < v_j) + #(i<j, v_i==v_j)
     double rr = 0.0, ri = 0.0;
+    int sr = 0, si = 0;
 #pragma unroll
     for (int j = 0; j < MAXL; ++j) {
       if (j < loops) {
@@ -337,8 +338,14 @@ __global__ __launch_bounds__(256) void estimate_kernel(
             pi += (vi[i] < vi[j]) || (i < j && vi[i] == vi[j]);
           }
         }
-        if (pr == med) rr = vr[j];
-        if (pi == med) ri = vi[j];
+        if (pr == med) {
+          rr = vr[j];
+          sr = j;
+        }
+        if (pi == med) {
+          ri = vi[j];
+          si = j;
+        }
       }
     }
     Node nd;
@@ -346,35 +353,41 @@ __global__ __launch_bounds__(256) void estimate_kernel(
     nd.value.x = rr;
     nd.value.y = ri;
     out[h] = nd;
+    if (sel) {
+      sel[2 * h] = (uint8_t)sr;
+      sel[2 * h + 1] = (uint8_t)si;
+    }
   }
 }
 
 template <typename Real>
 static hipError_t estimate_dispatch(const int* hits, const int* count, int capacity,
                                     const void* bins, int loops, int n, const int* ai_dev, int B,
-                                    const double2* win, Node* out, hipStream_t st) {
+                                    const double2* win, Node* out, uint8_t* sel,
+                                    hipStream_t st) {
   using T2 = typename CT<Real>::T2;
   const dim3 grid(512), block(256);
   if (loops <= 8)
     hipLaunchKernelGGL((estimate_kernel<Real, 8>), grid, block, 0, st, hits, count, capacity,
-                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out);
+                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out, sel);
   else if (loops <= 16)
     hipLaunchKernelGGL((estimate_kernel<Real, 16>), grid, block, 0, st, hits, count, capacity,
-                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out);
+                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out, sel);
   else
     hipLaunchKernelGGL((estimate_kernel<Real, 32>), grid, block, 0, st, hits, count, capacity,
-                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out);
+                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out, sel);
   return hipGetLastError();
 }
 
 hipError_t estimate_launch(int is_fp32, const int* hits, const int* count, int capacity,
                            const void* bins_f, int loops, int n, const int* ai_dev, int B,
-                           const double2* filter_win, Node* out, hipStream_t st) {
+                           const double2* filter_win, Node* out, uint8_t* sel,
+                           hipStream_t st) {
   if (is_fp32)
     return estimate_dispatch<float>(hits, count, capacity, bins_f, loops, n, ai_dev, B,
-                                    filter_win, out, st);
+                                    filter_win, out, sel, st);
   return estimate_dispatch<double>(hits, count, capacity, bins_f, loops, n, ai_dev, B,
-                                   filter_win, out, st);
+                                   filter_win, out, sel, st);
 }
 
 }  // namespace sfft

@@ -124,9 +124,26 @@ hipError_t hash_hits_launch(const HashArgs& h, hipStream_t st);
 int hash_blocks(int loops_loc, int K, long long m);   // m = n / B
 
 // ---- estimation ---------------------------------------------------------------------------
+// sel (optional, 2 bytes per hit): the loops whose values the real / imaginary lower medians
+// returned (stable-rank order)
 hipError_t estimate_launch(int is_fp32, const int* hits, const int* count, int capacity,
                            const void* bins_f, int loops, int n, const int* ai_dev, int B,
-                           const double2* filter_win, Node* out, hipStream_t st);
+                           const double2* filter_win, Node* out, uint8_t* sel,
+                           hipStream_t st);
+
+// ---- value map and its adjoint (FP64) -----------------------------------------------------
+// values[h] = Re v_{sel[2h]}(hits[h]) + i Im v_{sel[2h+1]}(hits[h]) from the bucket spectra
+hipError_t value_pick_launch(const int* hits, int nh, const uint8_t* sel,
+                             const double2* bins_f, int n, const int* ai_dev, int B,
+                             const double2* win, double2* values, hipStream_t st);
+// u[l][b] = the adjoint seeds of the (hit, part) entries landing in bucket b of loop l,
+// summed in entry order; head (loops*B ints) and next (2*nh ints) are scratch
+hipError_t adjoint_seed_launch(const int* hits, int nh, const uint8_t* sel, const double2* g,
+                               int n, const int* ai_dev, int B, int loops, const double2* win,
+                               int* head, int* next, double2* u, hipStream_t st);
+// gx[(i ai_l) mod n] += conj(f[i]) gb[l][bucket(i)], loop by loop (gx cleared first)
+hipError_t adjoint_taps_launch(const double2* f, const double2* gb, const int* ai_dev, int n,
+                               int B, int w, int loops, int tiled, double2* gx, hipStream_t st);
 
 // ---- small utilities ---------------------------------------------------------------------
 hipError_t fill_u32(void* p, uint32_t v, size_t words, hipStream_t st);

@@ -214,6 +214,9 @@ struct sfft_plan {
   int* count = nullptr;
   double2* win = nullptr;
   Node* nodes = nullptr;
+  uint8_t* sel = nullptr;       // median-selected loops of the last transform (2 per hit)
+  int* adj_head = nullptr;      // adjoint scratch: loops*B list heads, 2*cap links
+  int* adj_next = nullptr;
   const void* d_ft = nullptr;
   int has_filter = 0;
   int last_count = 0;
@@ -331,6 +334,9 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   items.push_back({(void**)&P->count, 16});
   items.push_back({(void**)&P->win, (size_t)P->m * 16});
   items.push_back({(void**)&P->nodes, (size_t)P->cap * sizeof(Node)});
+  items.push_back({(void**)&P->sel, (size_t)P->cap * 2});
+  items.push_back({(void**)&P->adj_head, (size_t)L * B * 4});
+  items.push_back({(void**)&P->adj_next, (size_t)P->cap * 2 * 4});
   size_t total = 0;
   for (auto& it : items) total = align_up(total, 256) + align_up(it.bytes, 256);
   if (hipMalloc(&P->ws, total) != hipSuccess) {
@@ -409,25 +415,18 @@ static sfft_status draws_check(const sfft_plan* P, const sfft_draws* d) {
     if ((x) != hipSuccess) return SFFT_ERR_HIP; \
   } while (0)
 
+static sfft_status upload_draws(sfft_plan* P, const sfft_draws* dr);
+
 extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const sfft_draws* dr,
                                           Node* d_nodes, int* d_count) {
   if (!P || !d_x) return SFFT_ERR_INVALID;
   if (!P->has_filter) return SFFT_ERR_NO_FILTER;
-  sfft_status v = draws_check(P, dr);
-  if (v != SFFT_OK) return v;
   const sfft_params& p = P->p;
   hipStream_t st = P->st;
   const int L = p.loops, Ll = p.loops_loc, Rl = p.RLFC_loops, K = p.B_thresh;
   const long long W = p.W_RLFC, B = p.B;
-  // draws: packed into pinned staging (after the previous upload out of it has completed),
-  // then one asynchronous copy
-  if (P->up_pending) HIPCK(hipEventSynchronize(P->up_ev));
-  memcpy(P->h_draws, dr->a, (size_t)L * 4);
-  memcpy(P->h_draws + L, dr->ai, (size_t)L * 4);
-  memcpy(P->h_draws + 2 * L, dr->offsets, (size_t)Rl * 4);
-  HIPCK(hipMemcpyAsync(P->d_a, P->h_draws, (size_t)(2 * L + Rl) * 4, hipMemcpyHostToDevice, st));
-  HIPCK(hipEventRecord(P->up_ev, st));
-  P->up_pending = 1;
+  sfft_status v = upload_draws(P, dr);
+  if (v != SFFT_OK) return v;
 
   HashArgs h;
   h.n = p.n;
@@ -526,8 +525,90 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
 
   // estimation
   HIPCK(estimate_launch(P->fp32, P->hits, h.count, P->cap, P->bins_f, L, p.n, P->d_ai, p.B,
-                        P->win, d_nodes ? d_nodes : P->nodes, st));
+                        P->win, d_nodes ? d_nodes : P->nodes, P->sel, st));
   HIPCK(mark(7));
+  return SFFT_OK;
+}
+
+// upload one transform's draws (after the previous upload out of the staging has completed)
+static sfft_status upload_draws(sfft_plan* P, const sfft_draws* dr) {
+  sfft_status v = draws_check(P, dr);
+  if (v != SFFT_OK) return v;
+  const int L = P->p.loops, Rl = P->p.RLFC_loops;
+  if (P->up_pending) HIPCK(hipEventSynchronize(P->up_ev));
+  memcpy(P->h_draws, dr->a, (size_t)L * 4);
+  memcpy(P->h_draws + L, dr->ai, (size_t)L * 4);
+  memcpy(P->h_draws + 2 * L, dr->offsets, (size_t)Rl * 4);
+  HIPCK(hipMemcpyAsync(P->d_a, P->h_draws, (size_t)(2 * L + Rl) * 4, hipMemcpyHostToDevice,
+                       P->st));
+  HIPCK(hipEventRecord(P->up_ev, P->st));
+  P->up_pending = 1;
+  return SFFT_OK;
+}
+
+static sfft_status value_map_check(const sfft_plan* P, int nh, const int* d_hits,
+                                   const unsigned char* d_sel) {
+  if (P->fp32) return SFFT_ERR_INVALID;                 // FP64 plans only
+  if (!P->has_filter) return SFFT_ERR_NO_FILTER;
+  if (nh < 0 || nh > P->cap || (nh > 0 && (!d_hits || !d_sel))) return SFFT_ERR_INVALID;
+  if (P->p.n / P->p.B > 8192) return SFFT_ERR_INVALID;  // bounds the per-bucket entry lists
+  return SFFT_OK;
+}
+
+extern "C" sfft_status sfft_value_apply(sfft_plan* P, const void* d_x, const sfft_draws* dr,
+                                        int nh, const int* d_hits, const unsigned char* d_sel,
+                                        complex_t* d_values) {
+  if (!P || !d_x || (nh > 0 && !d_values)) return SFFT_ERR_INVALID;
+  sfft_status v = value_map_check(P, nh, d_hits, d_sel);
+  if (v == SFFT_OK) v = upload_draws(P, dr);
+  if (v != SFFT_OK) return v;
+  const sfft_params& p = P->p;
+  const int tiled = (p.fold_mode == SFFT_FOLD_REFERENCE) && (p.n < (1 << 27));
+  HIPCK(perm_filter_launch(0, d_x, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, p.loops, tiled,
+                           P->st));
+  FftLaunch f;
+  f.tab = &P->tabB;
+  f.in = P->bins_t;
+  f.in_bstride = p.B;
+  f.out = P->bins_f;
+  f.out_bstride = p.B;
+  f.scratch = P->bins_scr;
+  f.batch = p.loops;
+  f.store = kStoreCplx;
+  HIPCK(fft_run(f, P->st));
+  HIPCK(value_pick_launch(d_hits, nh, d_sel, (const double2*)P->bins_f, p.n, P->d_ai, p.B,
+                          P->win, (double2*)d_values, P->st));
+  return SFFT_OK;
+}
+
+extern "C" sfft_status sfft_value_adjoint(sfft_plan* P, const complex_t* d_grad,
+                                          const sfft_draws* dr, int nh, const int* d_hits,
+                                          const unsigned char* d_sel, complex_t* d_grad_x) {
+  if (!P || !d_grad_x || (nh > 0 && !d_grad)) return SFFT_ERR_INVALID;
+  sfft_status v = value_map_check(P, nh, d_hits, d_sel);
+  if (v == SFFT_OK) v = upload_draws(P, dr);
+  if (v != SFFT_OK) return v;
+  const sfft_params& p = P->p;
+  hipError_t err = hipSuccess;
+  const FftTables* inv = cached_tables(P->log2B, 1, 0, &err);
+  if (!inv) return SFFT_ERR_HIP;
+  // 1. per-bucket seeds, 2. their unnormalised inverse FFT_B, 3. scatter through the taps
+  HIPCK(adjoint_seed_launch(d_hits, nh, d_sel, (const double2*)d_grad, p.n, P->d_ai, p.B,
+                            p.loops, P->win, P->adj_head, P->adj_next, (double2*)P->bins_t,
+                            P->st));
+  FftLaunch f;
+  f.tab = inv;
+  f.in = P->bins_t;
+  f.in_bstride = p.B;
+  f.out = P->bins_f;
+  f.out_bstride = p.B;
+  f.scratch = P->bins_scr;
+  f.batch = p.loops;
+  f.store = kStoreCplx;
+  HIPCK(fft_run(f, P->st));
+  const int tiled = (p.fold_mode == SFFT_FOLD_REFERENCE) && (p.n < (1 << 27));
+  HIPCK(adjoint_taps_launch((const double2*)P->d_ft, (const double2*)P->bins_f, P->d_ai, p.n,
+                            p.B, p.w, p.loops, tiled, (double2*)d_grad_x, P->st));
   return SFFT_OK;
 }
 
@@ -600,6 +681,7 @@ extern "C" const void* sfft_plan_debug_buffer(const sfft_plan* P, int which) {
     case 4: return P->bins_t;
     case 5: return P->nodes;
     case 6: return P->count;
+    case 7: return P->sel;
   }
   return nullptr;
 }

@@ -7,7 +7,7 @@ from ._lib import LIB_PATH, LibraryNotBuilt, lib  # noqa: F401
 from .core import (FOLD_ALL_TAPS, FOLD_REFERENCE, NODE_DTYPE, Plan, SfftError,  # noqa: F401
                    derive_params, dft_any, draw_permutations, fft, fft_plan_passes,
                    filter_taps, filter_window, floor_to_pow2, make_filter, metric,
-                   signal_spectrum, synthesize, to_complex64, truth_spectrum, twiddle,
-                   validate)
+                   signal_spectrum, sparse_fft, synthesize, to_complex64, truth_spectrum,
+                   twiddle, validate)
 
 __version__ = lib.sfft_version().decode()

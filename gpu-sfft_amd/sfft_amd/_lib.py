@@ -60,7 +60,7 @@ EXPORTED_FUNCTIONS = [
     "sfft_status_string", "sfft_params_validate", "sfft_plan_create", "sfft_plan_destroy",
     "sfft_plan_workspace_bytes", "sfft_plan_set_filter", "sfft_execute_async", "sfft_execute",
     "sfft_plan_set_profiling", "sfft_plan_stage_times", "sfft_plan_interval_times",
-    "sfft_plan_debug_buffer",
+    "sfft_plan_debug_buffer", "sfft_value_apply", "sfft_value_adjoint",
     "sfft_fft_device", "sfft_twiddle", "sfft_fft_plan_passes", "sfft_version",
     "sfft_device_count", "sfft_filter_taps", "sfft_make_filter", "sfft_filter_window",
     "sfft_dft_any_device", "sfft_convert_c128_to_c64", "sfft_convert_c64_to_c128",
@@ -112,6 +112,10 @@ def _declare(L):
     L.sfft_plan_interval_times.restype = _i
     L.sfft_plan_debug_buffer.argtypes = [_p, _i]
     L.sfft_plan_debug_buffer.restype = _p
+    L.sfft_value_apply.argtypes = [_p, _p, C.POINTER(Draws), _i, _p, _p, _p]
+    L.sfft_value_apply.restype = _i
+    L.sfft_value_adjoint.argtypes = [_p, _p, C.POINTER(Draws), _i, _p, _p, _p]
+    L.sfft_value_adjoint.restype = _i
     L.sfft_fft_device.argtypes = [_p, _p, _i, _i, _i, _i, _p]
     L.sfft_fft_device.restype = _i
     L.sfft_twiddle.argtypes = [C.c_uint64, C.c_uint64, _i, C.POINTER(_d), C.POINTER(_d)]

@@ -211,9 +211,63 @@ class Plan:
                                     _ptr(d_count) if d_count is not None else None)
         _check(st, "sfft_execute_async")
 
+    def selection(self, count):
+        """Device copies of the last transform's located indices (int32[count]) and
+        median-selected loops (uint8[count, 2]: real part, imaginary part)."""
+        torch = _torch()
+        hits = torch.empty(max(count, 1), dtype=torch.int32, device="cuda")[:count]
+        sel = torch.empty((max(count, 1), 2), dtype=torch.uint8, device="cuda")[:count]
+        torch.cuda.synchronize()
+        for t, which, nb in ((hits, 3, 4 * count), (sel, 7, 2 * count)):
+            if nb:
+                err = _hip().hipMemcpy(C.c_void_p(t.data_ptr()),
+                                       C.c_void_p(lib.sfft_plan_debug_buffer(self._h, which)),
+                                       C.c_size_t(nb), 3)
+                if err != 0:
+                    raise RuntimeError(f"hipMemcpy failed ({err})")
+        return hits, sel
+
+    def _value_args(self, hits, sel):
+        torch = _torch()
+        if hits.dtype != torch.int32 or sel.dtype != torch.uint8 or not hits.is_cuda or \
+                not sel.is_cuda or sel.numel() != 2 * hits.numel():
+            raise TypeError("hits must be int32 and sel uint8[nh, 2] device tensors")
+        return hits.contiguous(), sel.contiguous()
+
+    def value_apply(self, x, a, ai, offsets, hits, sel):
+        """The estimated values of the fixed hits/selections as a function of x (FP64 plans):
+        complex128[nh] device tensor."""
+        torch = _torch()
+        self._check_x(x)
+        hits, sel = self._value_args(hits, sel)
+        out = torch.empty(max(hits.numel(), 1), dtype=torch.complex128, device="cuda")
+        dr = self._draws(a, ai, offsets)
+        torch.cuda.synchronize()
+        _check(lib.sfft_value_apply(self._h, _ptr(x), C.byref(dr), hits.numel(), _ptr(hits),
+                                    _ptr(sel), _ptr(out)), "sfft_value_apply")
+        torch.cuda.synchronize()
+        return out[:hits.numel()]
+
+    def value_adjoint(self, grad, a, ai, offsets, hits, sel):
+        """The transpose of value_apply: grad (complex128[nh], dL/dRe + i dL/dIm of each
+        value) -> complex128[n] device tensor (dL/dRe x + i dL/dIm x)."""
+        torch = _torch()
+        hits, sel = self._value_args(hits, sel)
+        g = grad.to(torch.complex128).contiguous()
+        if g.numel() != hits.numel() or not g.is_cuda:
+            raise TypeError("grad must be a device tensor of nh entries")
+        gx = torch.empty(self.params["n"], dtype=torch.complex128, device="cuda")
+        dr = self._draws(a, ai, offsets)
+        torch.cuda.synchronize()
+        _check(lib.sfft_value_adjoint(self._h, _ptr(g) if g.numel() else None, C.byref(dr),
+                                      hits.numel(), _ptr(hits), _ptr(sel), _ptr(gx)),
+               "sfft_value_adjoint")
+        torch.cuda.synchronize()
+        return gx
+
     def debug_array(self, which, count, dtype):
         """Copies `count` items of a plan-owned device buffer to a numpy array
-        (0 bins_f, 1 J, 2 J2 lists, 3 hits, 4 bins_t, 5 nodes, 6 count)."""
+        (0 bins_f, 1 J, 2 J2 lists, 3 hits, 4 bins_t, 5 nodes, 6 count, 7 sel)."""
         torch = _torch()
         ptr = lib.sfft_plan_debug_buffer(self._h, which)
         out = np.empty(count, dtype=dtype)
@@ -346,3 +400,54 @@ def fft_plan_passes(log2n):
     out = (C.c_int * 8)()
     n = lib.sfft_fft_plan_passes(log2n, out)
     return [out[i] for i in range(n)]
+
+
+# ---- autograd ---------------------------------------------------------------------------------
+def _make_function():
+    torch = _torch()
+
+    class SparseFFTValues(torch.autograd.Function):
+        """One sparse FFT of x (complex128 device tensor) with an FP64 plan.  Returns the
+        estimated values (complex128[I_F], differentiable) and the located indices (int64,
+        not differentiable).  The backward pass holds the located indices and the
+        median-selected loops of the forward transform fixed and applies the adjoint of the
+        value map (sfft_value_adjoint) through the C ABI."""
+
+        @staticmethod
+        def forward(ctx, x, plan, a, ai, offsets):
+            if plan.fp32:
+                raise TypeError("the adjoint needs an fp64 plan")
+            nodes, cnt = plan.execute(x.detach(), a, ai, offsets)
+            hits, sel = plan.selection(cnt)
+            vals = torch.complex(torch.from_numpy(np.ascontiguousarray(nodes["re"])),
+                                 torch.from_numpy(np.ascontiguousarray(nodes["im"])))
+            keys = torch.from_numpy(nodes["key"].astype(np.int64)).to(x.device)
+            ctx.plan = plan
+            ctx.draws = (np.array(a, np.int32), np.array(ai, np.int32),
+                         np.array(offsets, np.int32))
+            ctx.save_for_backward(hits, sel)
+            ctx.mark_non_differentiable(keys)
+            return vals.to(x.device), keys
+
+        @staticmethod
+        def backward(ctx, g_vals, g_keys):
+            hits, sel = ctx.saved_tensors
+            if g_vals is None:
+                return None, None, None, None, None
+            gx = ctx.plan.value_adjoint(g_vals, *ctx.draws, hits, sel)
+            return gx, None, None, None, None
+
+    return SparseFFTValues
+
+
+_fn = None
+
+
+def sparse_fft(x, plan, a, ai, offsets):
+    """Differentiable sparse FFT: (values complex128[I_F], keys int64[I_F]) of x with the
+    given plan (FP64, filter set) and draws; gradients flow to x through the value map's
+    adjoint with the located indices and median choices held fixed."""
+    global _fn
+    if _fn is None:
+        _fn = _make_function()
+    return _fn.apply(x, plan, a, ai, offsets)

@@ -205,8 +205,29 @@ sfft_status sfft_plan_interval_times(sfft_plan *plan, double *seconds7);
 /* Debug/parity taps into the last transform (device pointers owned by the plan):
  * 0 bins_f (loops x B complex), 1 J (loops_loc x B_thresh int), 2 J2 lists (RLFC_loops x
  * B_thresh int), 3 hits (int, count from the last execute), 4 bins_t (loops x B complex),
- * 5 nodes (Node[hit_capacity]), 6 count (int). */
+ * 5 nodes (Node[hit_capacity]), 6 count (int), 7 sel (2 unsigned char per hit: the loops
+ * whose values the real and the imaginary lower medians returned). */
 const void *sfft_plan_debug_buffer(const sfft_plan *plan, int which);
+
+/* ---- the value map and its adjoint (FP64 plans) ----------------------------------------
+ * No counterpart in the reference: the gradient of the estimated values (innerLoop.cu:
+ * 396-466) with respect to the signal.  With nh located indices d_hits and their
+ * median-selected loops d_sel (2 per hit, as debug buffer 7 holds them after a transform)
+ * held fixed, each estimated value is a real-linear function of the signal:
+ *   values[h] = Re v_{sel[2h]}(hits[h]) + i Im v_{sel[2h+1]}(hits[h]),
+ *   v_l(I) = FFT_B(permute+filter of x, loop l)[bucket of I] * conj(F) / |F|^2.
+ * sfft_value_apply evaluates it for any device signal d_x (complex128[n]); applied to the
+ * transform's own signal it returns the transform's node values bit for bit.
+ * sfft_value_adjoint applies its transpose: d_grad[h] = dL/dRe values[h] + i dL/dIm values[h]
+ * (complex128[nh]) -> d_grad_x[m] = dL/dRe x[m] + i dL/dIm x[m] (complex128[n], overwritten).
+ * draws: the transform's draws (a, ai used).  Deterministic (no floating-point atomics).
+ * Both reuse the plan's bucket workspace (debug buffers 0 and 4 are overwritten).  They need
+ * n/B <= 8192 and return SFFT_ERR_INVALID for FP32 plans.  Asynchronous on the plan stream. */
+sfft_status sfft_value_apply(sfft_plan *plan, const void *d_x, const sfft_draws *draws, int nh,
+                             const int *d_hits, const unsigned char *d_sel, complex_t *d_values);
+sfft_status sfft_value_adjoint(sfft_plan *plan, const complex_t *d_grad, const sfft_draws *draws,
+                               int nh, const int *d_hits, const unsigned char *d_sel,
+                               complex_t *d_grad_x);
 
 /* ---- canonical FFT on the device (the same kernels the transform uses) ------------------ */
 /* In-place-or-not batched power-of-two FFT: count transforms of 2^log2n points, contiguous,

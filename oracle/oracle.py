@@ -375,3 +375,44 @@ def metric_dense(nodes, k, large_freq, n, x_f):
     lib().orc_metric_dense(_ptr(nodes), nodes.size, k, _ptr(lf), n, _ptr(xf), C.byref(me),
                            C.byref(mr), C.byref(err))
     return me.value, mr.value, err.value
+
+
+# ---- value map and its adjoint (project addition) -------------------------------------------
+def _plan_of(prm, w, fold_mode):
+    return Plan(n=prm["n"], B=prm["B"], B_thresh=prm["B_thresh"], W_RLFC=prm["W_RLFC"],
+                RLFC_loops=prm["RLFC_loops"], loops_thresh=prm["loops_thresh"],
+                loops_loc=prm["loops_loc"], loops=prm["loops"], w=int(w),
+                fold_mode=fold_mode, hit_capacity=524288)
+
+
+def estimate_select(hits, bins_f, loops, n, ai, B, filter_win):
+    """uint8[nh, 2]: the loops whose values the real / imaginary lower medians returned"""
+    hits = _i32(hits)
+    sel = np.zeros(2 * hits.size, dtype=np.uint8)
+    lib().orc_estimate_select(_ptr(hits), hits.size, _ptr(_c128(np.ravel(bins_f))), loops, n,
+                              _ptr(_i32(ai)), B, _ptr(_c128(filter_win)), _ptr(sel))
+    return sel.reshape(-1, 2)
+
+
+def value_apply64(prm, x, filter_time, filter_win, ai, hits, sel,
+                  fold_mode=FOLD_REFERENCE):
+    hits = _i32(hits)
+    sel = np.ascontiguousarray(sel, dtype=np.uint8).ravel()
+    out = np.zeros(hits.size, dtype=np.complex128)
+    pl = _plan_of(prm, np.size(filter_time), fold_mode)
+    lib().orc_value_apply64(C.byref(pl), _ptr(_c128(x)), _ptr(_c128(filter_time)),
+                            _ptr(_c128(filter_win)), _ptr(_i32(ai)), _ptr(hits), hits.size,
+                            _ptr(sel), _ptr(out))
+    return out
+
+
+def value_adjoint64(prm, filter_time, filter_win, ai, hits, sel, g,
+                    fold_mode=FOLD_REFERENCE):
+    hits = _i32(hits)
+    sel = np.ascontiguousarray(sel, dtype=np.uint8).ravel()
+    gx = np.zeros(prm["n"], dtype=np.complex128)
+    pl = _plan_of(prm, np.size(filter_time), fold_mode)
+    lib().orc_value_adjoint64(C.byref(pl), _ptr(_c128(filter_time)), _ptr(_c128(filter_win)),
+                              _ptr(_i32(ai)), _ptr(hits), hits.size, _ptr(sel),
+                              _ptr(_c128(g)), _ptr(gx))
+    return gx

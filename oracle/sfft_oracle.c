@@ -749,3 +749,154 @@ void orc_metric_dense(orc_node *nodes, int I_F, int k, const int *large_freq, in
   *missed_res = k - large_found;
   *err_total = err;
 }
+
+/* ------------------------------------------------------------------------------------------
+ * value map and its adjoint (a project addition, not in the reference)
+ *
+ * With the located indices I_h and, per hit, the loops whose values the real and the
+ * imaginary lower medians returned (sel[2h], sel[2h+1]) held fixed, the estimated values are
+ * a real-linear function of the signal:
+ *   yhat_h = Re(v_{sel[2h]}(I_h)) + i Im(v_{sel[2h+1]}(I_h)),
+ *   v_l(I) = bins_f[l][hashed_l(I)] conj(F) / |F|^2,  bins_f[l] = FFT_B(bins_t[l]),
+ *   bins_t[l][r] = sum of x[(i ai_l) mod n] f[i] over the taps i of bucket r,
+ * every step exactly as innerLoop.cu:396-466 / :725-773 compute it.  The adjoint maps a
+ * gradient g (g_h = dL/dRe yhat_h + i dL/dIm yhat_h) back to the signal:
+ *   u[l][h'] = sum over the (hit, part) entries landing in bucket h' of loop l, in hit order
+ *              (real part before imaginary part): (gr F.x/d, gr F.y/d) for the real part,
+ *              (-gi F.y/d, gi F.x/d) for the imaginary part  (d = |F|^2),
+ *   g_bins_t[l] = unnormalised inverse FFT_B(u[l])  (the conjugate transpose of FFT_B),
+ *   g_x[(i ai_l) mod n] += conj(f[i]) g_bins_t[l][r(i)], loop by loop, taps of the tiled
+ *              region (or all taps) first, then the reference fold's tail taps (Q2).
+ * pinned by: value_apply == the transform's node values (bit for bit), the adjoint identity
+ * <A x, y> = <x, A^T y> on random vectors, and the dense matrix of A on tiny n.
+ * ---------------------------------------------------------------------------------------- */
+static void est_term(int I, int l, int n, const int *ai, int B, const orc_c64 *win,
+                     long long *hashed_out, orc_c64 *F_out) {
+  const int m = n / B;
+  const long long nmask = (long long)n - 1;
+  const long long p = ((long long)ai[l] * I) % n;
+  long long hashed = p / m;
+  long long dist = p & (m - 1);
+  if (dist > m / 2) {
+    hashed = (hashed + 1) & (B - 1);
+    dist -= m;
+  }
+  dist = ((long long)n - dist) & nmask;
+  *hashed_out = hashed;
+  *F_out = win[(dist + m / 2) & nmask];
+}
+
+void orc_estimate_select(const int *hits, int I_F, const orc_c64 *bins_f, int loops, int n,
+                         const int *ai, int B, const orc_c64 *filter_win, unsigned char *sel) {
+  double *vr = (double *)malloc(sizeof(double) * (size_t)loops);
+  double *vi = (double *)malloc(sizeof(double) * (size_t)loops);
+  const int med = (loops - 1) / 2;
+  for (int h = 0; h < I_F; ++h) {
+    for (int l = 0; l < loops; ++l) {
+      long long hashed;
+      orc_c64 F;
+      est_term(hits[h], l, n, ai, B, filter_win, &hashed, &F);
+      const orc_c64 b = bins_f[(size_t)l * B + hashed];
+      const double abs2 = fma(F.x, F.x, F.y * F.y);
+      vr[l] = fma(b.x, F.x, b.y * F.y) / abs2;
+      vi[l] = fma(b.y, F.x, -(b.x * F.y)) / abs2;
+    }
+    /* the loop a stable insertion sort puts at position med */
+    for (int j = 0; j < loops; ++j) {
+      int pr = 0, pi = 0;
+      for (int i = 0; i < loops; ++i) {
+        pr += (vr[i] < vr[j]) || (i < j && vr[i] == vr[j]);
+        pi += (vi[i] < vi[j]) || (i < j && vi[i] == vi[j]);
+      }
+      if (pr == med) sel[2 * h] = (unsigned char)j;
+      if (pi == med) sel[2 * h + 1] = (unsigned char)j;
+    }
+  }
+  free(vr);
+  free(vi);
+}
+
+static void bins_of_signal(const orc_plan *p, const orc_c64 *x, const orc_c64 *f, const int *ai,
+                           orc_c64 *bins_f) {
+  const int B = p->B;
+  orc_c64 *scr = (orc_c64 *)malloc(sizeof(orc_c64) * (size_t)B);
+  for (int l = 0; l < p->loops; ++l) {
+    orc_c64 *row = bins_f + (size_t)l * B;
+    orc_perm_filter64(x, p->n, f, p->w, B, ai[l], p->fold_mode, row);
+    orc_fft64(row, ilog2_exact((uint64_t)B), 0, scr);
+  }
+  free(scr);
+}
+
+void orc_value_apply64(const orc_plan *p, const orc_c64 *x, const orc_c64 *f,
+                       const orc_c64 *win, const int *ai, const int *hits, int nh,
+                       const unsigned char *sel, orc_c64 *out) {
+  const int B = p->B;
+  orc_c64 *bins_f = (orc_c64 *)malloc(sizeof(orc_c64) * (size_t)p->loops * B);
+  bins_of_signal(p, x, f, ai, bins_f);
+  for (int h = 0; h < nh; ++h) {
+    long long hr, hi;
+    orc_c64 Fr, Fi;
+    est_term(hits[h], sel[2 * h], p->n, ai, B, win, &hr, &Fr);
+    est_term(hits[h], sel[2 * h + 1], p->n, ai, B, win, &hi, &Fi);
+    const orc_c64 br = bins_f[(size_t)sel[2 * h] * B + hr];
+    const orc_c64 bi = bins_f[(size_t)sel[2 * h + 1] * B + hi];
+    out[h].x = fma(br.x, Fr.x, br.y * Fr.y) / fma(Fr.x, Fr.x, Fr.y * Fr.y);
+    out[h].y = fma(bi.y, Fi.x, -(bi.x * Fi.y)) / fma(Fi.x, Fi.x, Fi.y * Fi.y);
+  }
+  free(bins_f);
+}
+
+void orc_value_adjoint64(const orc_plan *p, const orc_c64 *f, const orc_c64 *win,
+                         const int *ai, const int *hits, int nh, const unsigned char *sel,
+                         const orc_c64 *g, orc_c64 *gx) {
+  const int n = p->n, B = p->B, L = p->loops, w = p->w;
+  orc_c64 *u = (orc_c64 *)calloc((size_t)L * B, sizeof(orc_c64));
+  orc_c64 *scr = (orc_c64 *)malloc(sizeof(orc_c64) * (size_t)B);
+  for (int h = 0; h < nh; ++h) {
+    for (int part = 0; part < 2; ++part) {
+      const int l = sel[2 * h + part];
+      long long hb;
+      orc_c64 F;
+      est_term(hits[h], l, n, ai, B, win, &hb, &F);
+      const double d = fma(F.x, F.x, F.y * F.y);
+      orc_c64 *ub = &u[(size_t)l * B + hb];
+      if (part == 0) {
+        const double q = g[h].x / d;
+        ub->x = ub->x + q * F.x;
+        ub->y = ub->y + q * F.y;
+      } else {
+        const double q = g[h].y / d;
+        ub->x = ub->x - q * F.y;
+        ub->y = ub->y + q * F.x;
+      }
+    }
+  }
+  for (size_t i = 0; i < (size_t)n; ++i) gx[i].x = gx[i].y = 0.0;
+  const uint32_t mask = (uint32_t)n - 1u;
+  const int tiled = (p->fold_mode == ORC_FOLD_REFERENCE) && (n < (1 << 27));
+  const int tiles = w / B, residue = w % B;
+  for (int l = 0; l < L; ++l) {
+    orc_c64 *gb = u + (size_t)l * B;
+    orc_fft64(gb, ilog2_exact((uint64_t)B), 1, scr);
+    const uint32_t uai = (uint32_t)ai[l];
+    const int head = tiled ? tiles * B : w;
+    for (int i = 0; i < head; ++i) {
+      const orc_c64 fv = f[i], gv = gb[i & (B - 1)];
+      orc_c64 *dst = &gx[((uint32_t)i * uai) & mask];
+      dst->x = dst->x + fma(fv.x, gv.x, fv.y * gv.y);
+      dst->y = dst->y + fma(fv.x, gv.y, -(fv.y * gv.x));
+    }
+    if (tiled) {
+      for (int j = 0; j < residue; ++j) {
+        const int i = j * tiles + B;
+        const orc_c64 fv = f[i], gv = gb[j];
+        orc_c64 *dst = &gx[((uint32_t)i * uai) & mask];
+        dst->x = dst->x + fma(fv.x, gv.x, fv.y * gv.y);
+        dst->y = dst->y + fma(fv.x, gv.y, -(fv.y * gv.x));
+      }
+    }
+  }
+  free(u);
+  free(scr);
+}

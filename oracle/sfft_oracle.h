@@ -153,6 +153,21 @@ int orc_outer_loop32(const orc_plan *p, const orc_c32 *x, const orc_c32 *filter_
                      int *hits_out, orc_node *nodes_out, orc_c32 *bins_f_out, int *J_out,
                      int *J2_out, int *num_RLFC_out);
 
+/* ---- value map and its adjoint (project addition; see sfft_oracle.c) ---------------------
+ * sel[2h], sel[2h+1]: the loops whose values the real / imaginary lower medians of hit h
+ * returned (stable-rank order, as the reference's insertion sort orders ties).
+ * value_apply: the estimated values of the fixed hits as a function of x.
+ * value_adjoint: its transpose; g[h] = dL/dRe yhat_h + i dL/dIm yhat_h, gx[n] is overwritten.
+ * pinned by: apply == transform node values; <A x, y> = <x, A^T y>; dense matrix on tiny n. */
+void orc_estimate_select(const int *hits, int I_F, const orc_c64 *bins_f, int loops, int n,
+                         const int *ai, int B, const orc_c64 *filter_win, unsigned char *sel);
+void orc_value_apply64(const orc_plan *p, const orc_c64 *x, const orc_c64 *f,
+                       const orc_c64 *win, const int *ai, const int *hits, int nh,
+                       const unsigned char *sel, orc_c64 *out);
+void orc_value_adjoint64(const orc_plan *p, const orc_c64 *f, const orc_c64 *win,
+                         const int *ai, const int *hits, int nh, const unsigned char *sel,
+                         const orc_c64 *g, orc_c64 *gx);
+
 /* ---- harness metric (run_experiment, cuExperiment.cu:181-221) ---------------------------
  * qsort by |value| descending (comp_struct, utils.c:231-235), keep the first k in a dense
  * n-vector, MISSED = (k - FOUND, k - large_found) with FOUND by key search over all I_F
