@@ -338,8 +338,10 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.log2N = log2N;
     A.r = r;
     A.log2Lprev = log2Lprev;
+    // columns per tile: as many as fit the tile, up to qmax (does not change the arithmetic)
+    const int qmax = (L.qmax > 0 && L.qmax <= FFT_QMAX) ? L.qmax : FFT_QMAX;
     int log2Q = 0;
-    while ((1 << (log2Q + 1)) <= FFT_QMAX && ((1 << (log2Q + 1)) << r) <= FFT_CAP) ++log2Q;
+    while ((1 << (log2Q + 1)) <= qmax && ((1 << (log2Q + 1)) << r) <= FFT_CAP) ++log2Q;
     int tiles;
     if (p == 0) {
       const int lnq = log2N - log2Lcur;
@@ -420,17 +422,18 @@ constexpr int PF_NT = 256;
 // bins[l*B + r] = sum over the taps id of bucket r (canonical order) of v_l(id) * f[id], where
 // v_l(id) = x[(id * ai_l) mod n] gathered from the signal (TAPS = false), or read from the
 // tap-ordered staging array T[l*w + id] that the streaming pass filled (TAPS = true; same
-// values, coalesced reads).  Each thread owns one bucket of LPT loops.
+// values, coalesced reads).  One thread owns bucket r of LPT loops l0 .. l0+LPT-1.  Every
+// launch shape (single transform, batch, any LPT or block size) runs this body, so the sums
+// are the same bits whatever the shape.
 template <typename Real, int LPT, bool TAPS>
-__global__ __launch_bounds__(PF_NT) void perm_filter_kernel(
-    const typename CT<Real>::T2* __restrict__ x, const typename CT<Real>::T2* __restrict__ f,
-    typename CT<Real>::T2* __restrict__ bins, const int* __restrict__ ai_dev, uint32_t mask,
-    int B, int w, int loops, int tiled) {
+__device__ __forceinline__ void pf_body(const typename CT<Real>::T2* __restrict__ x,
+                                        const typename CT<Real>::T2* __restrict__ f,
+                                        typename CT<Real>::T2* __restrict__ bins,
+                                        const int* __restrict__ ai_dev, uint32_t mask, int B,
+                                        int w, int loops, int tiled, int r, int l0) {
   using T2 = typename CT<Real>::T2;
   using AC = typename AccOf<Real>::A;
-  const int r = blockIdx.x * PF_NT + threadIdx.x;
   if (r >= B) return;
-  const int l0 = blockIdx.y * LPT;
   uint32_t ai[LPT];
   const T2* __restrict__ src[LPT];
   AC acc[LPT];
@@ -462,6 +465,15 @@ __global__ __launch_bounds__(PF_NT) void perm_filter_kernel(
     if (l0 + li < loops) bins[(long long)(l0 + li) * B + r] = acc[li].get();
 }
 
+template <typename Real, int LPT, bool TAPS>
+__global__ __launch_bounds__(PF_NT) void perm_filter_kernel(
+    const typename CT<Real>::T2* __restrict__ x, const typename CT<Real>::T2* __restrict__ f,
+    typename CT<Real>::T2* __restrict__ bins, const int* __restrict__ ai_dev, uint32_t mask,
+    int B, int w, int loops, int tiled) {
+  pf_body<Real, LPT, TAPS>(x, f, bins, ai_dev, mask, B, w, loops, tiled,
+                           blockIdx.x * PF_NT + threadIdx.x, blockIdx.y * LPT);
+}
+
 template <bool TAPS>
 static hipError_t pf_launch(int is_fp32, const void* src, const void* f, void* bins,
                             const int* ai_dev, int n, int B, int w, int loops, int tiled,
@@ -478,6 +490,68 @@ static hipError_t pf_launch(int is_fp32, const void* src, const void* f, void* b
                        (const double2*)src, (const double2*)f, (double2*)bins, ai_dev, mask, B,
                        w, loops, tiled);
   return hipGetLastError();
+}
+
+// Batch: the permute+filter of every signal of a batch in one persistent launch.  Work item
+// i of job j (item0_j <= i < item0_{j+1}) is bucket block i % nbx of loop group i / nbx;
+// the workgroups walk the items with a grid stride (every workgroup runs the same number of
+// items +-1, whatever the signal sizes).
+template <typename Real, int LPT, int NT>
+__global__ __launch_bounds__(NT) void perm_filter_batch_kernel(const PfJob* __restrict__ jobs,
+                                                               int njobs, int total) {
+  using T2 = typename CT<Real>::T2;
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].item0 <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const PfJob& J = jobs[lo];
+    const int local = item - J.item0;
+    const int bx = local % J.nbx, gy = local / J.nbx;
+    pf_body<Real, LPT, false>((const T2*)J.x, (const T2*)J.f, (T2*)J.bins, J.ai, J.mask, J.B,
+                              J.w, J.loops, J.tiled, bx * NT + threadIdx.x, gy * LPT);
+  }
+}
+
+int pf_batch_items(int B, int loops, int lpt, int nt, int* nbx) {
+  *nbx = (B + nt - 1) / nt;
+  return *nbx * ((loops + lpt - 1) / lpt);
+}
+
+template <typename Real, int LPT>
+static hipError_t pfb_nt(const PfJob* jobs, int njobs, int total, int nt, int grid,
+                         hipStream_t st) {
+  if (nt == 128)
+    hipLaunchKernelGGL((perm_filter_batch_kernel<Real, LPT, 128>), dim3(grid), dim3(128), 0, st,
+                       jobs, njobs, total);
+  else if (nt == 256)
+    hipLaunchKernelGGL((perm_filter_batch_kernel<Real, LPT, 256>), dim3(grid), dim3(256), 0, st,
+                       jobs, njobs, total);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <typename Real>
+static hipError_t pfb_lpt(const PfJob* jobs, int njobs, int total, int lpt, int nt, int grid,
+                          hipStream_t st) {
+  switch (lpt) {
+    case 1: return pfb_nt<Real, 1>(jobs, njobs, total, nt, grid, st);
+    case 2: return pfb_nt<Real, 2>(jobs, njobs, total, nt, grid, st);
+    case 3: return pfb_nt<Real, 3>(jobs, njobs, total, nt, grid, st);
+    case 4: return pfb_nt<Real, 4>(jobs, njobs, total, nt, grid, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t perm_filter_batch_launch(int is_fp32, const PfJob* d_jobs, int njobs, int total,
+                                    int lpt, int nt, int grid, hipStream_t st) {
+  if (njobs <= 0 || total <= 0) return hipSuccess;
+  if (grid > total) grid = total;
+  return is_fp32 ? pfb_lpt<float>(d_jobs, njobs, total, lpt, nt, grid, st)
+                 : pfb_lpt<double>(d_jobs, njobs, total, lpt, nt, grid, st);
 }
 
 hipError_t perm_filter_launch(int is_fp32, const void* x, const void* f, void* bins,

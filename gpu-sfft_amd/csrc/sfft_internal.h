@@ -61,6 +61,7 @@ struct FftLaunch {
   const void* comb_x = nullptr;
   const int* comb_offs = nullptr;
   int comb_sigma = 1;
+  int qmax = 0;                  // most columns per tile (power of two; 0 = the default)
 };
 hipError_t fft_run(const FftLaunch& L, hipStream_t st);
 
@@ -70,6 +71,21 @@ hipError_t perm_filter_launch(int is_fp32, const void* x, const void* f, void* b
                               const int* ai_dev, int n, int B, int w, int loops, int tiled,
                               hipStream_t st);
 // same fold, reading the tap values from the staging array T[l*w + i] of the streaming pass
+// batch: one persistent launch over the permute+filter of several signals
+struct PfJob {
+  const void* x;       // signal (complex128 / complex64)
+  const void* f;       // taps
+  void* bins;          // loops x B folded buckets
+  const int* ai;       // device ai[loops]
+  uint32_t mask;       // n - 1
+  int B, w, loops, tiled;
+  int nbx;             // bucket blocks of nt threads
+  int item0;           // first work item of this signal
+};
+// items of one signal for a (loops per thread, threads per block) shape; *nbx = bucket blocks
+int pf_batch_items(int B, int loops, int lpt, int nt, int* nbx);
+hipError_t perm_filter_batch_launch(int is_fp32, const PfJob* d_jobs, int njobs, int total,
+                                    int lpt, int nt, int grid, hipStream_t st);
 hipError_t fold_taps_launch(int is_fp32, const void* T, const void* f, void* bins,
                             const int* ai_dev, int n, int B, int w, int loops, int tiled,
                             hipStream_t st);

@@ -174,62 +174,7 @@ const FftTables* cached_tables(int log2n, int inverse, int fp32, hipError_t* err
 
 using namespace sfft;
 
-// ============================================================================================
-// plan
-// ============================================================================================
-struct sfft_plan {
-  sfft_params p;
-  int fp32 = 0;
-  hipStream_t st = nullptr;
-  int log2n = 0, log2B = 0, log2W = 0, m = 0;
-  int bpe = 1;
-  int cap = MAXSIZE_HITS;
-  FftTables tabB, tabW;
-  void* ws = nullptr;
-  size_t ws_bytes = 0;
-  int* d_a = nullptr;
-  int* d_ai = nullptr;
-  int* d_offs = nullptr;
-  void* comb0 = nullptr;
-  void* comb1 = nullptr;
-  void* comb_keys = nullptr;
-  int stream = 0;               // streaming front end (else gathers)
-  void* taps = nullptr;         // loops x w tap values (stream)
-  void* comb_y = nullptr;       // RLFC_loops x W comb samples (stream)
-  void* bins_t = nullptr;
-  void* bins_scr = nullptr;
-  void* bins_f = nullptr;
-  void* bkeys = nullptr;
-  int* J = nullptr;
-  int* J2 = nullptr;
-  SelectWork selW, selB;
-  unsigned* bitmap = nullptr;
-  int bm_words = 0;
-  int* maxfirst = nullptr;
-  unsigned* vote = nullptr;
-  int* hitoff = nullptr;
-  int* blocksum = nullptr;
-  int* blockoff = nullptr;
-  int* hits = nullptr;
-  int* count = nullptr;
-  double2* win = nullptr;
-  Node* nodes = nullptr;
-  uint8_t* sel = nullptr;       // median-selected loops of the last transform (2 per hit)
-  int* adj_head = nullptr;      // adjoint scratch: loops*B list heads, 2*cap links
-  int* adj_next = nullptr;
-  const void* d_ft = nullptr;
-  int has_filter = 0;
-  int last_count = 0;
-  // pinned host staging: draws (a, ai, offsets packed like d_a..d_offs) and the hit count
-  int* h_draws = nullptr;
-  int* h_count = nullptr;
-  hipEvent_t up_ev = nullptr;   // last upload out of h_draws
-  int up_pending = 0;
-  // optional stage timing (hipEvents at the reference's stage boundaries)
-  int profile = 0;
-  hipEvent_t ev[9] = {nullptr, nullptr, nullptr, nullptr, nullptr,
-                      nullptr, nullptr, nullptr, nullptr};
-};
+#include "sfft_plan_impl.h"
 
 static int ilog2_exact(long long v) {
   if (v <= 0) return -1;
@@ -396,7 +341,7 @@ extern "C" sfft_status sfft_plan_set_filter(sfft_plan* P, const void* d_filter_t
   return SFFT_OK;
 }
 
-static sfft_status draws_check(const sfft_plan* P, const sfft_draws* d) {
+sfft_status draws_check(const sfft_plan* P, const sfft_draws* d) {
   if (!d || !d->a || !d->ai || !d->offsets) return SFFT_ERR_INVALID;
   const long long n = P->p.n;
   const int sigma = P->p.n / P->p.W_RLFC;
@@ -410,35 +355,19 @@ static sfft_status draws_check(const sfft_plan* P, const sfft_draws* d) {
   return SFFT_OK;
 }
 
-#define HIPCK(x)                            \
-  do {                                      \
-    if ((x) != hipSuccess) return SFFT_ERR_HIP; \
-  } while (0)
-
-static sfft_status upload_draws(sfft_plan* P, const sfft_draws* dr);
-
-extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const sfft_draws* dr,
-                                          Node* d_nodes, int* d_count) {
-  if (!P || !d_x) return SFFT_ERR_INVALID;
-  if (!P->has_filter) return SFFT_ERR_NO_FILTER;
+HashArgs hash_args(const sfft_plan* P, int* d_count) {
   const sfft_params& p = P->p;
-  hipStream_t st = P->st;
-  const int L = p.loops, Ll = p.loops_loc, Rl = p.RLFC_loops, K = p.B_thresh;
-  const long long W = p.W_RLFC, B = p.B;
-  sfft_status v = upload_draws(P, dr);
-  if (v != SFFT_OK) return v;
-
   HashArgs h;
   h.n = p.n;
   h.B = p.B;
-  h.K = K;
+  h.K = p.B_thresh;
   h.W = p.W_RLFC;
-  h.loops_loc = Ll;
+  h.loops_loc = p.loops_loc;
   h.loops_thresh = p.loops_thresh;
   h.bpe = P->bpe;
   h.J = P->J;
   h.J2 = P->J2;
-  h.nJ2 = Rl * K;
+  h.nJ2 = p.RLFC_loops * p.B_thresh;
   h.a_dev = P->d_a;
   h.ai_dev = P->d_ai;
   h.bitmap = P->bitmap;
@@ -451,19 +380,30 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   h.hits = P->hits;
   h.capacity = P->cap;
   h.count = d_count ? d_count : P->count;
+  return h;
+}
+
+// The transform in three stage groups, each enqueued on an explicit stream (the plan's own
+// for a single transform; a batch spreads its signals over several streams and runs the
+// permute+filter of all signals as one launch).  The stage-boundary events of the profiling
+// mode are recorded by the front and back groups when `marks` is set.
+// front: vote/bitmap clear, streaming pass (option), comb FFT + cutoff + approved bitmaps
+sfft_status run_front(sfft_plan* P, const void* d_x, const HashArgs& h, hipStream_t st,
+                             bool marks) {
+  const sfft_params& p = P->p;
+  const int L = p.loops, Rl = p.RLFC_loops, K = p.B_thresh;
+  const long long W = p.W_RLFC;
   auto mark = [&](int i) -> hipError_t {
-    return P->profile ? hipEventRecord(P->ev[i], st) : hipSuccess;
+    return (marks && P->profile) ? hipEventRecord(P->ev[i], st) : hipSuccess;
   };
   HIPCK(mark(0));
   HIPCK(hash_clear_launch(h, st));
   HIPCK(mark(1));
-
   // streaming front end: one pass over x fills the tap staging array and the comb samples
   if (P->stream)
     HIPCK(stream_scatter_launch(P->fp32, d_x, P->taps, P->comb_y, P->d_a, P->d_offs, p.n, p.w,
                                 L, Rl, p.W_RLFC, st));
   HIPCK(mark(2));
-
   // RLFC: comb samples (gathered in pass 0, or from the streaming pass) + W-point FFT, keys
   // of |Y|^2 out of the last pass
   FftLaunch c;
@@ -476,6 +416,7 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   c.key_batches = Rl;
   c.batch = Rl;
   c.store = kStoreKeys;
+  c.qmax = P->qmax_W;
   if (P->stream) {
     c.in = P->comb_y;
     c.in_bstride = W;
@@ -491,15 +432,31 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   HIPCK(select_topk(P->fp32, P->comb_keys, W, (int)W, K, Rl, P->J2, 0, P->selW, st));
   HIPCK(hash_bitmap_launch(h, st));
   HIPCK(mark(3));
+  return SFFT_OK;
+}
 
-  // permutation + filtering of every loop, then the batched bucket FFT
+// fold: permutation + filtering of every loop into bins_t
+sfft_status run_fold(sfft_plan* P, const void* d_x, hipStream_t st) {
+  const sfft_params& p = P->p;
   const int tiled = (p.fold_mode == SFFT_FOLD_REFERENCE) && (p.n < (1 << 27));
   if (P->stream)
-    HIPCK(fold_taps_launch(P->fp32, P->taps, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, L,
+    HIPCK(fold_taps_launch(P->fp32, P->taps, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, p.loops,
                            tiled, st));
   else
-    HIPCK(perm_filter_launch(P->fp32, d_x, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, L,
+    HIPCK(perm_filter_launch(P->fp32, d_x, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, p.loops,
                              tiled, st));
+  return SFFT_OK;
+}
+
+// back: bucket FFT + cutoff, reverse hash, estimation
+sfft_status run_back(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t st,
+                            bool marks) {
+  const sfft_params& p = P->p;
+  const int L = p.loops, Ll = p.loops_loc, K = p.B_thresh;
+  const long long B = p.B;
+  auto mark = [&](int i) -> hipError_t {
+    return (marks && P->profile) ? hipEventRecord(P->ev[i], st) : hipSuccess;
+  };
   HIPCK(mark(4));
   FftLaunch f;
   f.tab = &P->tabB;
@@ -513,16 +470,15 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   f.key_batches = Ll;
   f.batch = L;
   f.store = kStoreBoth;
+  f.qmax = P->qmax_B;
   HIPCK(fft_run(f, st));
   // ascending bucket lists: the hit list is ordered by (loop, bucket, walk)
   HIPCK(select_topk(P->fp32, P->bkeys, B, (int)B, K, Ll, P->J, 1, P->selB, st));
   HIPCK(mark(5));
-
   // reverse hash: votes, then the hit list in (loop, bucket, walk) order
   HIPCK(hash_vote_launch(h, st));
   HIPCK(hash_hits_launch(h, st));
   HIPCK(mark(6));
-
   // estimation
   HIPCK(estimate_launch(P->fp32, P->hits, h.count, P->cap, P->bins_f, L, p.n, P->d_ai, p.B,
                         P->win, d_nodes ? d_nodes : P->nodes, P->sel, st));
@@ -530,8 +486,20 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   return SFFT_OK;
 }
 
+extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const sfft_draws* dr,
+                                          Node* d_nodes, int* d_count) {
+  if (!P || !d_x) return SFFT_ERR_INVALID;
+  if (!P->has_filter) return SFFT_ERR_NO_FILTER;
+  sfft_status v = upload_draws(P, dr);
+  if (v != SFFT_OK) return v;
+  const HashArgs h = hash_args(P, d_count);
+  if ((v = run_front(P, d_x, h, P->st, true)) != SFFT_OK) return v;
+  if ((v = run_fold(P, d_x, P->st)) != SFFT_OK) return v;
+  return run_back(P, h, d_nodes, P->st, true);
+}
+
 // upload one transform's draws (after the previous upload out of the staging has completed)
-static sfft_status upload_draws(sfft_plan* P, const sfft_draws* dr) {
+sfft_status upload_draws(sfft_plan* P, const sfft_draws* dr) {
   sfft_status v = draws_check(P, dr);
   if (v != SFFT_OK) return v;
   const int L = P->p.loops, Rl = P->p.RLFC_loops;

@@ -1,0 +1,79 @@
+// sfft_plan_impl.h -- the plan object and its stage groups, shared by the single-transform
+// API (sfft_plan.hip) and the batch executor (sfft_batch.hip).  Internal: not installed.
+#pragma once
+#include "sfft_internal.h"
+
+using namespace sfft;
+
+// plan: one transform configuration and its workspace
+struct sfft_plan {
+  sfft_params p;
+  int fp32 = 0;
+  hipStream_t st = nullptr;
+  int log2n = 0, log2B = 0, log2W = 0, m = 0;
+  int bpe = 1;
+  int cap = MAXSIZE_HITS;
+  FftTables tabB, tabW;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  int* d_a = nullptr;
+  int* d_ai = nullptr;
+  int* d_offs = nullptr;
+  void* comb0 = nullptr;
+  void* comb1 = nullptr;
+  void* comb_keys = nullptr;
+  int stream = 0;               // streaming front end (else gathers)
+  void* taps = nullptr;         // loops x w tap values (stream)
+  void* comb_y = nullptr;       // RLFC_loops x W comb samples (stream)
+  void* bins_t = nullptr;
+  void* bins_scr = nullptr;
+  void* bins_f = nullptr;
+  void* bkeys = nullptr;
+  int* J = nullptr;
+  int* J2 = nullptr;
+  SelectWork selW, selB;
+  unsigned* bitmap = nullptr;
+  int bm_words = 0;
+  int* maxfirst = nullptr;
+  unsigned* vote = nullptr;
+  int* hitoff = nullptr;
+  int* blocksum = nullptr;
+  int* blockoff = nullptr;
+  int* hits = nullptr;
+  int* count = nullptr;
+  double2* win = nullptr;
+  Node* nodes = nullptr;
+  uint8_t* sel = nullptr;       // median-selected loops of the last transform (2 per hit)
+  int qmax_W = 0, qmax_B = 0;   // FFT columns per tile (0 = default), set by the autotuner
+  int* adj_head = nullptr;      // adjoint scratch: loops*B list heads, 2*cap links
+  int* adj_next = nullptr;
+  const void* d_ft = nullptr;
+  int has_filter = 0;
+  int last_count = 0;
+  // pinned host staging: draws (a, ai, offsets packed like d_a..d_offs) and the hit count
+  int* h_draws = nullptr;
+  int* h_count = nullptr;
+  hipEvent_t up_ev = nullptr;   // last upload out of h_draws
+  int up_pending = 0;
+  // optional stage timing (hipEvents at the reference's stage boundaries)
+  int profile = 0;
+  hipEvent_t ev[9] = {nullptr, nullptr, nullptr, nullptr, nullptr,
+                      nullptr, nullptr, nullptr, nullptr};
+};
+
+
+#define HIPCK(x)                                \
+  do {                                          \
+    if ((x) != hipSuccess) return SFFT_ERR_HIP; \
+  } while (0)
+
+sfft_status draws_check(const sfft_plan* P, const sfft_draws* d);
+// packs the draws into the plan's pinned staging and copies them to the device on its stream
+sfft_status upload_draws(sfft_plan* P, const sfft_draws* dr);
+HashArgs hash_args(const sfft_plan* P, int* d_count);
+// the transform in three stage groups on an explicit stream (marks: record the plan's
+// profiling events)
+sfft_status run_front(sfft_plan* P, const void* d_x, const HashArgs& h, hipStream_t st,
+                      bool marks);
+sfft_status run_fold(sfft_plan* P, const void* d_x, hipStream_t st);
+sfft_status run_back(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t st, bool marks);

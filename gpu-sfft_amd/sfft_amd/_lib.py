@@ -61,6 +61,9 @@ EXPORTED_FUNCTIONS = [
     "sfft_plan_workspace_bytes", "sfft_plan_set_filter", "sfft_execute_async", "sfft_execute",
     "sfft_plan_set_profiling", "sfft_plan_stage_times", "sfft_plan_interval_times",
     "sfft_plan_debug_buffer", "sfft_value_apply", "sfft_value_adjoint",
+    "sfft_batch_create", "sfft_batch_destroy", "sfft_batch_plan", "sfft_batch_set_filter",
+    "sfft_batch_set_graph", "sfft_batch_execute_async", "sfft_batch_execute",
+    "sfft_batch_counts_device", "sfft_batch_tune", "sfft_batch_set_shape",
     "sfft_fft_device", "sfft_twiddle", "sfft_fft_plan_passes", "sfft_version",
     "sfft_device_count", "sfft_filter_taps", "sfft_make_filter", "sfft_filter_window",
     "sfft_dft_any_device", "sfft_convert_c128_to_c64", "sfft_convert_c64_to_c128",
@@ -116,6 +119,26 @@ def _declare(L):
     L.sfft_value_apply.restype = _i
     L.sfft_value_adjoint.argtypes = [_p, _p, C.POINTER(Draws), _i, _p, _p, _p]
     L.sfft_value_adjoint.restype = _i
+    L.sfft_batch_create.argtypes = [C.POINTER(Params), _i, _i, _p, C.POINTER(_p)]
+    L.sfft_batch_create.restype = _i
+    L.sfft_batch_destroy.argtypes = [_p]
+    L.sfft_batch_destroy.restype = None
+    L.sfft_batch_plan.argtypes = [_p, _i]
+    L.sfft_batch_plan.restype = _p
+    L.sfft_batch_set_filter.argtypes = [_p, _i, _p, _p]
+    L.sfft_batch_set_filter.restype = _i
+    L.sfft_batch_set_graph.argtypes = [_p, _i]
+    L.sfft_batch_set_graph.restype = _i
+    L.sfft_batch_execute_async.argtypes = [_p, _p, C.POINTER(Draws)]
+    L.sfft_batch_execute_async.restype = _i
+    L.sfft_batch_execute.argtypes = [_p, _p, C.POINTER(Draws), _p, _p]
+    L.sfft_batch_execute.restype = _i
+    L.sfft_batch_counts_device.argtypes = [_p]
+    L.sfft_batch_counts_device.restype = _p
+    L.sfft_batch_tune.argtypes = [_p, _p, C.POINTER(Draws), _i, _p]
+    L.sfft_batch_tune.restype = _i
+    L.sfft_batch_set_shape.argtypes = [_p, _i, _i, _i, _i, _i]
+    L.sfft_batch_set_shape.restype = _i
     L.sfft_fft_device.argtypes = [_p, _p, _i, _i, _i, _i, _p]
     L.sfft_fft_device.restype = _i
     L.sfft_twiddle.argtypes = [C.c_uint64, C.c_uint64, _i, C.POINTER(_d), C.POINTER(_d)]

@@ -278,6 +278,125 @@ class Plan:
         return out
 
 
+# ---- batches of uneven signals -----------------------------------------------------------------
+def batch_workload(count, seed, q_range=(18, 24), k_range=(50, 4000), Bcst=1.0, loops_loc=2,
+                   loops_thresh=2, loops_est=5, RLFC_cst=256, RLFC_loops=2):
+    """Per-signal parameters of a batch of uneven signals: q uniform on q_range (inclusive),
+    k log-uniform on k_range.  A draw that violates the reference's asserts (experiment.c:
+    77-84; e.g. q=18 gives B=4096, so k >= 2048 breaks B_thresh < B) or this library's plan
+    limits is redrawn (k only, q kept)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        q = int(rng.integers(q_range[0], q_range[1] + 1))
+        for _try in range(1000):
+            k = int(round(math.exp(rng.uniform(math.log(k_range[0]), math.log(k_range[1])))))
+            prm = derive_params(q, k, Bcst, loops_loc, loops_thresh, loops_est, RLFC_cst,
+                                RLFC_loops)
+            if prm["assert_ok"] and validate(prm) == 0:
+                break
+        else:
+            raise ValueError(f"no k in {k_range} gives valid parameters at q={q}")
+        out.append(prm)
+    return out
+
+
+class Batch:
+    """Several signals of uneven n and k transformed together (sfft_batch_*): one persistent
+    permute+filter launch for all signals, the other stages on worker streams, optional
+    hipGraph capture and an autotuner over bit-equivalent launch shapes."""
+
+    def __init__(self, params_list, precision="fp32", fold_mode=FOLD_REFERENCE):
+        self.params = [dict(p) for p in params_list]
+        self.count = len(self.params)
+        self.fp32 = precision == "fp32"
+        arr = (L.Params * self.count)(*[_cparams(p, fold_mode, 0, "gather")
+                                       for p in self.params])
+        h = C.c_void_p()
+        _check(lib.sfft_batch_create(arr, self.count, L.SFFT_FP32 if self.fp32 else L.SFFT_FP64,
+                                     None, C.byref(h)), "sfft_batch_create")
+        self._h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.sfft_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_filter(self, i, filter_time, filter_win):
+        win = np.ascontiguousarray(filter_win, dtype=np.complex128)
+        self._keep[("ft", i)] = filter_time
+        _check(lib.sfft_batch_set_filter(self._h, i, _ptr(filter_time), _np_ptr(win)),
+               "sfft_batch_set_filter")
+
+    def set_graph(self, on=True):
+        _check(lib.sfft_batch_set_graph(self._h, int(on)), "sfft_batch_set_graph")
+
+    def set_shape(self, lpt, block, grid, qmax_bucket=0, qmax_comb=0):
+        _check(lib.sfft_batch_set_shape(self._h, lpt, block, grid, qmax_bucket, qmax_comb),
+               "sfft_batch_set_shape")
+
+    def _args(self, xs, draws):
+        torch = _torch()
+        want = torch.complex64 if self.fp32 else torch.complex128
+        for x, p in zip(xs, self.params):
+            if x.dtype != want or not x.is_cuda or x.numel() != p["n"] or not x.is_contiguous():
+                raise TypeError(f"each signal must be a contiguous {want} device tensor of n "
+                                "entries")
+        ptrs = (C.c_void_p * self.count)(*[x.data_ptr() for x in xs])
+        keep, dr = [], (L.Draws * self.count)()
+        ip = C.POINTER(C.c_int)
+        for i, (a, ai, off) in enumerate(draws):
+            a, ai, off = (np.ascontiguousarray(v, dtype=np.int32) for v in (a, ai, off))
+            keep.append((a, ai, off))
+            dr[i] = L.Draws(a.ctypes.data_as(ip), ai.ctypes.data_as(ip), off.ctypes.data_as(ip))
+        return ptrs, dr, keep
+
+    def execute_async(self, xs, draws):
+        torch = _torch()
+        ptrs, dr, keep = self._args(xs, draws)
+        torch.cuda.synchronize()
+        _check(lib.sfft_batch_execute_async(self._h, ptrs, dr), "sfft_batch_execute_async")
+        self._keep["draws"] = keep
+
+    def execute(self, xs, draws):
+        """Runs the batch; returns a list of (nodes numpy structured array, I_F) per signal."""
+        torch = _torch()
+        ptrs, dr, keep = self._args(xs, draws)
+        counts = np.zeros(self.count, dtype=np.int32)
+        torch.cuda.synchronize()
+        _check(lib.sfft_batch_execute(self._h, ptrs, dr, None, _np_ptr(counts)),
+               "sfft_batch_execute")
+        out = []
+        for i in range(self.count):
+            cnt = int(counts[i])
+            nodes = np.empty(cnt, dtype=NODE_DTYPE)
+            if cnt:
+                src = lib.sfft_plan_debug_buffer(lib.sfft_batch_plan(self._h, i), 5)
+                err = _hip().hipMemcpy(_np_ptr(nodes), C.c_void_p(src), C.c_size_t(nodes.nbytes),
+                                       2)
+                if err != 0:
+                    raise RuntimeError(f"hipMemcpy failed ({err})")
+            out.append((nodes, cnt))
+        return out
+
+    def tune(self, xs, draws, reps=3):
+        """Times the bit-equivalent launch shapes on this batch and keeps the fastest."""
+        torch = _torch()
+        ptrs, dr, keep = self._args(xs, draws)
+        ch = np.zeros(4, dtype=np.int32)
+        torch.cuda.synchronize()
+        _check(lib.sfft_batch_tune(self._h, ptrs, dr, reps, _np_ptr(ch)), "sfft_batch_tune")
+        return {"loops_per_thread": int(ch[0]), "block": int(ch[1]), "grid": int(ch[2]),
+                "bucket_fft_columns": int(ch[3] & 0xff), "comb_fft_columns": int(ch[3] >> 8)}
+
+
 _hip_lib = None
 
 

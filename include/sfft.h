@@ -229,6 +229,42 @@ sfft_status sfft_value_adjoint(sfft_plan *plan, const complex_t *d_grad, const s
                                int nh, const int *d_hits, const unsigned char *d_sel,
                                complex_t *d_grad_x);
 
+/* ---- batches of uneven signals -----------------------------------------------------------
+ * No counterpart in the reference (which transforms one signal per outer_loop call): count
+ * signals, each with its own parameters (n, k, B, ... may all differ), transformed together.
+ * The permute+filter of all signals runs as one persistent launch; every signal's other
+ * stages run on worker streams; with sfft_batch_set_graph(b, 1) the whole launch sequence is
+ * captured into a hipGraph once and replayed (re-captured when the signal pointers, filters
+ * or shapes change).  Each signal's results are bit-identical to sfft_execute of the same
+ * signal with the same plan parameters and draws.  stream NULL: the batch makes its own. */
+typedef struct sfft_batch sfft_batch;
+sfft_status sfft_batch_create(const sfft_params *params, int count, sfft_precision prec,
+                              void *stream, sfft_batch **out);
+void sfft_batch_destroy(sfft_batch *b);
+/* the plan of signal i (its debug buffers hold signal i's last transform) */
+sfft_plan *sfft_batch_plan(sfft_batch *b, int i);
+sfft_status sfft_batch_set_filter(sfft_batch *b, int i, const void *d_filter_time,
+                                  const complex_t *h_filter_win);
+sfft_status sfft_batch_set_graph(sfft_batch *b, int on);
+/* d_x[count] device signals, draws[count]; results stay on the device (per-signal plan
+ * buffers; counts in sfft_batch_counts_device) */
+sfft_status sfft_batch_execute_async(sfft_batch *b, const void *const *d_x,
+                                     const sfft_draws *draws);
+/* same, then h_counts[count] and, where h_nodes[i] is not NULL, the nodes of signal i
+ * (h_nodes[i] holds hit_capacity entries); synchronous */
+sfft_status sfft_batch_execute(sfft_batch *b, const void *const *d_x, const sfft_draws *draws,
+                               Node *const *h_nodes, int *h_counts);
+const int *sfft_batch_counts_device(const sfft_batch *b);
+/* Autotuner: times bit-equivalent launch shapes on this batch (reps runs each) and keeps the
+ * fastest: permute+filter loops per thread {1..4} x block {128, 256} x persistent grid
+ * {1024, 2048, 4096}, and FFT columns per tile {1..16} of the bucket and comb FFTs.
+ * choice4 (optional): lpt, block, grid, bucket columns | comb columns << 8. */
+sfft_status sfft_batch_tune(sfft_batch *b, const void *const *d_x, const sfft_draws *draws,
+                            int reps, int *choice4);
+/* sets a shape explicitly (lpt 1..4, block 128/256, grid >= 1, columns 0..16, 0 = default) */
+sfft_status sfft_batch_set_shape(sfft_batch *b, int lpt, int block, int grid, int qmax_bucket,
+                                 int qmax_comb);
+
 /* ---- canonical FFT on the device (the same kernels the transform uses) ------------------ */
 /* In-place-or-not batched power-of-two FFT: count transforms of 2^log2n points, contiguous,
  * complex128 (SFFT_FP64) or complex64 (SFFT_FP32).  Unnormalised, forward = exp(-2 pi i). */
