@@ -75,9 +75,23 @@ __device__ __forceinline__ void bfly4(T2& a0, T2& a1, T2& a2, T2& a3, int invers
 // FFT pass
 // ============================================================================================
 constexpr int FFT_NT = 512;
-constexpr int FFT_CAP = 8192;                // complex elements per tile (LDS)
-constexpr int FFT_EPT = FFT_CAP / FFT_NT;    // elements per thread
 constexpr int FFT_QMAX = 16;                 // columns per tile
+// complex elements per tile (LDS): 8192 complex64 (66 KB) or 4096 complex128 (66 KB), so
+// that two workgroups share a CU and one loads while the other computes
+template <typename Real>
+struct Tile {
+  static constexpr int cap = sizeof(Real) == 8 ? 4096 : 8192;
+  static constexpr int ept = cap / FFT_NT;   // elements per thread
+};
+static int tile_cap(int is_fp32) { return is_fp32 ? Tile<float>::cap : Tile<double>::cap; }
+
+// Workgroups are dealt to the 8 XCDs round-robin by launch order.  Tiles t, t+1, ... read
+// neighbouring columns of the same 128-byte lines, so give the workgroups of one XCD (ids
+// x, x+8, x+16, ...) consecutive tiles: the lines are then shared in that XCD's L2 instead
+// of being fetched once per XCD.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
 
 struct PassArgs {
   const void* in;
@@ -111,7 +125,7 @@ __device__ __forceinline__ T2 outer_tw(const PassArgs& A, uint32_t e, int log2Lc
 // stores and the column-interleaved loads spread over the LDS banks.
 #define LPAD(x) ((x) + ((x) >> 4))
 
-template <typename T2>
+template <typename T2, int EPT>
 __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol, int log2ns,
                                            const T2* __restrict__ twR, int inverse) {
   const int tid = threadIdx.x;
@@ -119,9 +133,9 @@ __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol,
   const int nbf = (1 << log2ncol) * Rq;
   const int ns = 1 << log2ns;
   const int tstep = 1 << (r - log2ns - 2);
-  T2 v[FFT_EPT];
+  T2 v[EPT];
 #pragma unroll
-  for (int kk = 0; kk < FFT_EPT / 4; ++kk) {
+  for (int kk = 0; kk < EPT / 4; ++kk) {
     const int bi = tid + kk * FFT_NT;
     if (bi < nbf) {
       const int c = bi >> (r - 2);
@@ -144,7 +158,7 @@ __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol,
   }
   __syncthreads();
 #pragma unroll
-  for (int kk = 0; kk < FFT_EPT / 4; ++kk) {
+  for (int kk = 0; kk < EPT / 4; ++kk) {
     const int bi = tid + kk * FFT_NT;
     if (bi < nbf) {
       const int c = bi >> (r - 2);
@@ -160,7 +174,7 @@ __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol,
   __syncthreads();
 }
 
-template <typename T2>
+template <typename T2, int EPT>
 __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol, int log2ns,
                                            const T2* __restrict__ twR) {
   const int tid = threadIdx.x;
@@ -168,9 +182,9 @@ __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol,
   const int nbf = (1 << log2ncol) * Rh;
   const int ns = 1 << log2ns;
   const int tstep = 1 << (r - log2ns - 1);
-  T2 v[FFT_EPT];
+  T2 v[EPT];
 #pragma unroll
-  for (int kk = 0; kk < FFT_EPT / 2; ++kk) {
+  for (int kk = 0; kk < EPT / 2; ++kk) {
     const int bi = tid + kk * FFT_NT;
     if (bi < nbf) {
       const int c = bi >> (r - 1);
@@ -184,7 +198,7 @@ __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol,
   }
   __syncthreads();
 #pragma unroll
-  for (int kk = 0; kk < FFT_EPT / 2; ++kk) {
+  for (int kk = 0; kk < EPT / 2; ++kk) {
     const int bi = tid + kk * FFT_NT;
     if (bi < nbf) {
       const int c = bi >> (r - 1);
@@ -202,8 +216,10 @@ template <typename Real, int LOAD, int STORE>
 __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
   using T2 = typename CT<Real>::T2;
   using KT = typename CT<Real>::K;
-  __shared__ T2 lds[FFT_CAP + FFT_CAP / 16 + FFT_QMAX];
+  constexpr int CAP = Tile<Real>::cap, EPT = Tile<Real>::ept;
+  __shared__ T2 lds[CAP + CAP / 16 + FFT_QMAX];
   const int tid = threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const long long b = blockIdx.y;
   const int r = A.r;
   const int R = 1 << r;
@@ -217,12 +233,12 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
   long long q0 = 0, q = 0, t0 = 0;
   if (first) {
     log2ncol = min(A.log2Q, A.log2N - log2Lcur);
-    q0 = (long long)blockIdx.x << log2ncol;
+    q0 = (long long)tile << log2ncol;
   } else {
     log2ncol = A.log2Q;
     const int log2tpq = log2Lprev - A.log2Q;
-    q = (long long)blockIdx.x >> log2tpq;
-    t0 = ((long long)blockIdx.x & ((1LL << log2tpq) - 1)) << A.log2Q;
+    q = (long long)tile >> log2tpq;
+    t0 = ((long long)tile & ((1LL << log2tpq) - 1)) << A.log2Q;
   }
   const int ncol = 1 << log2ncol;
   const int TE = ncol << r;
@@ -230,7 +246,7 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
   // ---- load (+ outer twiddle) into LDS, column c at lds[c*RP + v]
   const T2* __restrict__ in = (const T2*)A.in + b * A.in_bstride;
 #pragma unroll
-  for (int k = 0; k < FFT_EPT; ++k) {
+  for (int k = 0; k < EPT; ++k) {
     const int e = tid + k * FFT_NT;
     if (e < TE) {
       const int c = e & (ncol - 1);
@@ -256,18 +272,18 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
   int log2ns = 0;
   int rem = r;
   while (rem >= 2) {
-    fft_stage4<T2>(lds, RP, r, log2ncol, log2ns, twR, A.inverse);
+    fft_stage4<T2, EPT>(lds, RP, r, log2ncol, log2ns, twR, A.inverse);
     log2ns += 2;
     rem -= 2;
   }
-  if (rem == 1) fft_stage2<T2>(lds, RP, r, log2ncol, log2ns, twR);
+  if (rem == 1) fft_stage2<T2, EPT>(lds, RP, r, log2ncol, log2ns, twR);
 
   // ---- store: out[q*Lcur + t + Lprev*k]
   T2* __restrict__ out = (T2*)A.out + b * A.out_bstride;
   KT* __restrict__ keys = (KT*)A.keys + b * A.keys_bstride;
   const bool wkeys = (STORE != kStoreCplx) && (b < A.key_batches);
 #pragma unroll
-  for (int k = 0; k < FFT_EPT; ++k) {
+  for (int k = 0; k < EPT; ++k) {
     const int e = tid + k * FFT_NT;
     if (e < TE) {
       int c, kk;
@@ -340,8 +356,9 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.log2Lprev = log2Lprev;
     // columns per tile: as many as fit the tile, up to qmax (does not change the arithmetic)
     const int qmax = (L.qmax > 0 && L.qmax <= FFT_QMAX) ? L.qmax : FFT_QMAX;
+    const int cap = tile_cap(T.is_fp32);
     int log2Q = 0;
-    while ((1 << (log2Q + 1)) <= qmax && ((1 << (log2Q + 1)) << r) <= FFT_CAP) ++log2Q;
+    while ((1 << (log2Q + 1)) <= qmax && ((1 << (log2Q + 1)) << r) <= cap) ++log2Q;
     int tiles;
     if (p == 0) {
       const int lnq = log2N - log2Lcur;

@@ -98,11 +98,16 @@ hipError_t stream_scatter_launch(int is_fp32, const void* x, void* T, void* y, c
 // ---- top-K select over `segs` key segments of N keys (stride seg_stride) ----------------
 struct SelectWork {
   void* state = nullptr;      // per segment: threshold, top digit, counters
-  unsigned* hist = nullptr;   // segs * 4096 (zero between calls)
+  unsigned* hist = nullptr;   // segs * 4096 column sums, then segs * nchunks * 4096 rows
   unsigned* done = nullptr;   // segs (zero between calls)
   int* cnt = nullptr;         // segs * nchunks
   void* cand_key = nullptr;   // segs * N keys
   int* cand_idx = nullptr;    // segs * N
+  int* emit_idx = nullptr;    // segs * N (per-block regions)
+  void* blk_key = nullptr;    // segs * N candidate keys in per-block regions
+  int* blk_idx = nullptr;     // segs * N
+  int* ccnt = nullptr;        // segs * nchunks candidate counts
+  int* ecnt = nullptr;        // segs * nchunks emitted counts
   int max_segs = 0;
   int max_chunks = 0;
 };

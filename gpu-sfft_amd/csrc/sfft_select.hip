@@ -43,19 +43,19 @@ struct KeyBits<unsigned int> {
 
 __device__ __forceinline__ int ilog2_dev(int N) { return 31 - __clz(N); }
 
-constexpr int SEL_LDS_CAND = 4096;   // per-block LDS list capacity in the split pass
 
-// elements per block: about 128 blocks per segment for large N, at least 2048
+// elements per block: about 256 blocks per segment for large N, at least 2048
 static int sel_chunk(int N) {
   int c = 2048;
-  while ((long long)c * 128 < N) c <<= 1;
+  while ((long long)c * 256 < N) c <<= 1;
   return c;
 }
 
 size_t select_work_bytes(int segs, int N) {
   const int nch = (N + sel_chunk(N) - 1) / sel_chunk(N);
-  return (size_t)segs * sizeof(SelState) + (size_t)segs * SEL_BINS * 4 + (size_t)segs * 4 +
-         (size_t)segs * nch * 4 + (size_t)segs * N * 8 + (size_t)segs * N * 4 + 1024;
+  return (size_t)segs * sizeof(SelState) + (size_t)segs * (nch + 1) * SEL_BINS * 4 +
+         (size_t)segs * 4 + (size_t)segs * nch * 4 * 3 + (size_t)segs * N * 8 * 2 +
+         (size_t)segs * N * 4 * 3 + 4096;
 }
 
 void select_work_bind(SelectWork* w, void* base, int segs, int N) {
@@ -67,11 +67,16 @@ void select_work_bind(SelectWork* w, void* base, int segs, int N) {
     return r;
   };
   w->state = take((size_t)segs * sizeof(SelState));
-  w->hist = (unsigned*)take((size_t)segs * SEL_BINS * 4);
+  w->hist = (unsigned*)take((size_t)segs * (nch + 1) * SEL_BINS * 4);
   w->done = (unsigned*)take((size_t)segs * 4);
   w->cnt = (int*)take((size_t)segs * nch * 4);
   w->cand_key = take((size_t)segs * N * 8);
   w->cand_idx = (int*)take((size_t)segs * N * 4);
+  w->emit_idx = (int*)take((size_t)segs * N * 4);
+  w->blk_key = take((size_t)segs * N * 8);
+  w->blk_idx = (int*)take((size_t)segs * N * 4);
+  w->ccnt = (int*)take((size_t)segs * nch * 4);
+  w->ecnt = (int*)take((size_t)segs * nch * 4);
   w->max_segs = segs;
   w->max_chunks = nch;
 }
@@ -132,17 +137,13 @@ __device__ __forceinline__ u128 composite(K key, unsigned idx, int N, int IB) {
   return ((u128)km << IB) | (u128)(unsigned)((unsigned)(N - 1) - idx);
 }
 
-// ---- 1. top-digit histogram; last block picks d0 -----------------------------------------
+// ---- 1. top-digit histogram: one row of SEL_BINS counts per block (plain stores) ---------
 template <typename K>
 __global__ __launch_bounds__(SEL_NT) void sel_hist0_kernel(const K* __restrict__ keys,
                                                            long long seg_stride, int N, int chunk,
-                                                           int nchunks, int Ksel,
-                                                           SelState* __restrict__ st,
-                                                           unsigned* __restrict__ ghist,
-                                                           unsigned* __restrict__ done) {
+                                                           int nchunks,
+                                                           unsigned* __restrict__ rows) {
   __shared__ unsigned h[SEL_BINS];
-  __shared__ int s_last;
-  __shared__ int sh[16];
   const int tid = threadIdx.x, seg = blockIdx.y;
   for (int i = tid; i < SEL_BINS; i += SEL_NT) h[i] = 0u;
   __syncthreads();
@@ -163,61 +164,82 @@ __global__ __launch_bounds__(SEL_NT) void sel_hist0_kernel(const K* __restrict__
     }
   }
   __syncthreads();
-  unsigned* gh = ghist + (size_t)seg * SEL_BINS;
-  for (int i = tid; i < SEL_BINS; i += SEL_NT)
-    if (h[i]) atomicAdd(&gh[i], h[i]);
-  __threadfence();
+  unsigned* row = rows + ((size_t)seg * nchunks + blockIdx.x) * SEL_BINS;
+  for (int i = tid; i < SEL_BINS; i += SEL_NT) row[i] = h[i];
+}
+
+// ---- 1b. column sums of the block rows ----------------------------------------------------
+// a block sums 16 bins, 16 threads per bin each over every 16th row, then reduces in LDS
+constexpr int PICK_NT = 256;
+constexpr int PICK_BINS = 16;
+__global__ __launch_bounds__(PICK_NT) void sel_colsum_kernel(int nchunks,
+                                                             const unsigned* __restrict__ rows,
+                                                             unsigned* __restrict__ ghist) {
+  __shared__ unsigned part[PICK_NT / PICK_BINS][PICK_BINS];
+  const int tid = threadIdx.x, seg = blockIdx.y;
+  const int bl = tid % PICK_BINS, g = tid / PICK_BINS;
+  constexpr int G = PICK_NT / PICK_BINS;
+  const int bin0 = blockIdx.x * PICK_BINS;
+  const unsigned* r = rows + (size_t)seg * nchunks * SEL_BINS + bin0 + bl;
+  unsigned a = 0u;
+#pragma unroll 4
+  for (int c = g; c < nchunks; c += G) a += r[(size_t)c * SEL_BINS];
+  part[g][bl] = a;
   __syncthreads();
-  if (tid == 0) s_last = (atomicAdd(&done[seg], 1u) == (unsigned)(nchunks - 1));
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  // bins from the top: thread t owns ranks [8t, 8t+8)
-  unsigned v[8];
+  if (tid < PICK_BINS) {
+    unsigned acc = 0u;
+#pragma unroll
+    for (int j = 0; j < G; ++j) acc += part[j][tid];
+    ghist[(size_t)seg * SEL_BINS + bin0 + tid] = acc;
+  }
+}
+
+// ---- 1c. pick the top digit d0 holding the K-th largest (one block per segment) -----------
+__global__ __launch_bounds__(PICK_NT) void sel_pick_kernel(int Ksel,
+                                                           const unsigned* __restrict__ ghist,
+                                                           SelState* __restrict__ st) {
+  __shared__ int sh[16];
+  const int tid = threadIdx.x, seg = blockIdx.x;
+  const unsigned* gh = ghist + (size_t)seg * SEL_BINS;
+  // bins from the top: thread t owns ranks [16t, 16t+16)
+  constexpr int PER = SEL_BINS / PICK_NT;
+  unsigned v[PER];
   int S = 0;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int rank = 8 * tid + j;
-    v[j] = __hip_atomic_load(&gh[SEL_BINS - 1 - rank], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int j = 0; j < PER; ++j) {
+    v[j] = gh[SEL_BINS - 1 - (PER * tid + j)];
     S += (int)v[j];
   }
   const int E = block_excl_scan_s(S, sh);
   if (E < Ksel && Ksel <= E + S) {
     int cum = E;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < PER; ++j) {
       if (cum + (int)v[j] >= Ksel) {
-        st[seg].d0 = SEL_BINS - 1 - (8 * tid + j);
+        st[seg].d0 = SEL_BINS - 1 - (PER * tid + j);
         st[seg].need0 = Ksel - cum;
         break;
       }
       cum += (int)v[j];
     }
   }
-  for (int i = tid; i < SEL_BINS; i += SEL_NT) gh[i] = 0u;
-  if (tid == 0) {
-    done[seg] = 0u;
-    st[seg].cand_count = 0;
-    st[seg].out_count = 0;
-  }
 }
 
-// ---- 2. split: emit digit > d0 (unordered mode), collect digit == d0 ---------------------
+// ---- 2. split: collect digit == d0 (candidates), and digit > d0 (emitted, unordered mode) --
+// Block b writes its candidates to [b*chunk, b*chunk + count) of the segment's candidate
+// arrays, its emitted indices to the same range of the emitted array, and the two counts to
+// ccnt/ecnt[seg][b]: no global atomics (the refine pass compacts).
 template <typename K>
 __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__ keys,
                                                            long long seg_stride, int N, int chunk,
-                                                           int Ksel, int emit,
-                                                           SelState* __restrict__ st,
+                                                           int nchunks, int emit,
+                                                           const SelState* __restrict__ st,
                                                            K* __restrict__ ckey,
                                                            int* __restrict__ cidx,
-                                                           int* __restrict__ out) {
-  // candidates and emitted indices are gathered in LDS (one LDS atomic per wave per round)
-  // and reserved in global memory with one atomic per block; a block that outgrows the LDS
-  // lists (degenerate inputs) appends the rest directly
-  __shared__ K lk[SEL_LDS_CAND];
-  __shared__ int li[SEL_LDS_CAND];
-  __shared__ int le[SEL_LDS_CAND];
-  __shared__ int s_nc, s_ne, s_gc, s_ge;
+                                                           int* __restrict__ eidx,
+                                                           int* __restrict__ ccnt,
+                                                           int* __restrict__ ecnt) {
+  __shared__ int s_nc, s_ne;
   const int tid = threadIdx.x, seg = blockIdx.y;
   if (tid == 0) {
     s_nc = 0;
@@ -226,9 +248,10 @@ __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__
   __syncthreads();
   const unsigned d0 = (unsigned)st[seg].d0;
   const K* kp = keys + (long long)seg * seg_stride;
-  K* ck = ckey + (long long)seg * N;
-  int* ci = cidx + (long long)seg * N;
-  int* o = out + (long long)seg * Ksel;
+  const long long rb = (long long)seg * N + (long long)blockIdx.x * chunk;
+  K* ck = ckey + rb;
+  int* ci = cidx + rb;
+  int* ce = eidx + rb;
   const int begin = blockIdx.x * chunk, end = min(N, begin + chunk);
   for (int base = begin; base < end; base += SEL_NT) {
     const int i = base + tid;
@@ -238,41 +261,73 @@ __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__
     const bool isc = ok && d == d0;
     const int pc = wave_append(&s_nc, isc);
     if (isc) {
-      if (pc < SEL_LDS_CAND) {
-        lk[pc] = k;
-        li[pc] = i;
-      } else {
-        const int g = atomicAdd(&st[seg].cand_count, 1);
-        ck[g] = k;
-        ci[g] = i;
-      }
+      ck[pc] = k;
+      ci[pc] = i;
     }
     if (emit) {
       const bool ise = ok && d > d0;
       const int pe = wave_append(&s_ne, ise);
-      if (ise) {
-        if (pe < SEL_LDS_CAND) {
-          le[pe] = i;
-        } else {
-          const int g = atomicAdd(&st[seg].out_count, 1);
-          if (g < Ksel) o[g] = i;
-        }
-      }
+      if (ise) ce[pe] = i;
     }
   }
   __syncthreads();
-  const int nc = min(s_nc, SEL_LDS_CAND), ne = min(s_ne, SEL_LDS_CAND);
   if (tid == 0) {
-    s_gc = nc ? atomicAdd(&st[seg].cand_count, nc) : 0;
-    s_ge = ne ? atomicAdd(&st[seg].out_count, ne) : 0;
+    ccnt[(long long)seg * nchunks + blockIdx.x] = s_nc;
+    ecnt[(long long)seg * nchunks + blockIdx.x] = s_ne;
+  }
+}
+
+constexpr int SEL_MAX_CHUNKS = 1024;   // blocks per segment
+constexpr int CMP_NT = 256;
+
+// ---- 2b. compaction: block b of the split moves its candidates to the contiguous list at
+// the sum of the counts of blocks < b, and (unordered mode) its emitted indices to out --------
+template <typename K>
+__global__ __launch_bounds__(CMP_NT) void sel_compact_kernel(int N, int Ksel, int chunk,
+                                                             int nchunks, int emit,
+                                                             SelState* __restrict__ st,
+                                                             const K* __restrict__ bkey,
+                                                             const int* __restrict__ bidx,
+                                                             const int* __restrict__ eidx,
+                                                             const int* __restrict__ ccnt,
+                                                             const int* __restrict__ ecnt,
+                                                             K* __restrict__ ckey,
+                                                             int* __restrict__ cidx,
+                                                             int* __restrict__ out) {
+  __shared__ int sh[16];
+  const int tid = threadIdx.x, seg = blockIdx.y, b = blockIdx.x;
+  const int* cc = ccnt + (long long)seg * nchunks;
+  const int* ec = ecnt + (long long)seg * nchunks;
+  int pc = 0, pe = 0;
+  for (int c = tid; c < b; c += CMP_NT) {
+    pc += cc[c];
+    pe += ec[c];
+  }
+  const int oc = block_excl_scan_s(pc, sh), oe = block_excl_scan_s(pe, sh);
+  // the block totals: exclusive scan of the last thread plus its own value
+  __shared__ int s_oc, s_oe;
+  if (tid == CMP_NT - 1) {
+    s_oc = oc + pc;
+    s_oe = oe + pe;
   }
   __syncthreads();
-  for (int j = tid; j < nc; j += SEL_NT) {
-    ck[s_gc + j] = lk[j];
-    ci[s_gc + j] = li[j];
+  const int offc = s_oc, offe = s_oe;
+  const int nc = cc[b], ne = ec[b];
+  const long long src = (long long)seg * N + (long long)b * chunk;
+  const long long dst = (long long)seg * N + offc;
+  for (int j = tid; j < nc; j += CMP_NT) {
+    ckey[dst + j] = bkey[src + j];
+    cidx[dst + j] = bidx[src + j];
   }
-  for (int j = tid; j < ne; j += SEL_NT)
-    if (s_ge + j < Ksel) o[s_ge + j] = le[j];
+  if (emit) {
+    int* o = out + (long long)seg * Ksel;
+    for (int j = tid; j < ne; j += CMP_NT)
+      if (offe + j < Ksel) o[offe + j] = eidx[src + j];
+  }
+  if (b == nchunks - 1 && tid == 0) {
+    st[seg].cand_count = offc + nc;
+    st[seg].out_count = offe + ne;
+  }
 }
 
 // ---- 3. refine over the candidates (one workgroup per segment) ----------------------------
@@ -285,12 +340,14 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
   __shared__ unsigned h[SEL_BINS];
   __shared__ int sh[16];
   __shared__ int s_digit, s_above, s_exit;
+  __shared__ int s_out;
   const int tid = threadIdx.x, seg = blockIdx.x;
   const int IB = ilog2_dev(N);
   const int TB = KeyBits<K>::KB + IB;
   const int nc = st[seg].cand_count;
   const K* ck = ckey + (long long)seg * N;
   const int* ci = cidx + (long long)seg * N;
+  if (tid == 0) s_out = 0;
   int r = st[seg].need0;
   u128 prefix = (u128)(unsigned)st[seg].d0;   // the top SEL_DIG bits of c
   int consumed = SEL_DIG;
@@ -352,11 +409,13 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
     st[seg].thr_lo = (unsigned long long)T;
   }
   if (emit) {
+    // after the emitted indices (placed by the compaction): the candidates >= T
     int* o = out + (long long)seg * Ksel;
+    const int ne = st[seg].out_count;
     for (int base = 0; base < nc; base += 1024) {
       const int j = base + tid;
       const bool sel = j < nc && composite<K>(ck[j], (unsigned)ci[j], N, IB) >= T;
-      const int po = wave_append(&st[seg].out_count, sel);
+      const int po = ne + wave_append(&s_out, sel);
       if (sel && po < Ksel) o[po] = ci[j];
     }
   }
@@ -434,12 +493,25 @@ static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ks
                               int* out, int ordered, const SelectWork& w, hipStream_t st) {
   const int chunk = sel_chunk(N);
   const int nchunks = (N + chunk - 1) / chunk;
+  if (nchunks > SEL_MAX_CHUNKS || nchunks > w.max_chunks || segs > w.max_segs)
+    return hipErrorInvalidValue;
   dim3 grid(nchunks, segs);
   SelState* S = (SelState*)w.state;
+  // work layout: column sums, then the per-block histogram rows
+  unsigned* rows = w.hist + (size_t)segs * SEL_BINS;
   hipLaunchKernelGGL((sel_hist0_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
-                     chunk, nchunks, Ksel, S, w.hist, w.done);
+                     chunk, nchunks, rows);
+  hipLaunchKernelGGL(sel_colsum_kernel, dim3(SEL_BINS / PICK_BINS, segs), dim3(PICK_NT), 0, st,
+                     nchunks, (const unsigned*)rows, w.hist);
+  hipLaunchKernelGGL(sel_pick_kernel, dim3(segs), dim3(PICK_NT), 0, st, Ksel,
+                     (const unsigned*)w.hist, S);
   hipLaunchKernelGGL((sel_split_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
-                     chunk, Ksel, ordered ? 0 : 1, S, (K*)w.cand_key, w.cand_idx, out);
+                     chunk, nchunks, ordered ? 0 : 1, (const SelState*)S, (K*)w.blk_key,
+                     w.blk_idx, w.emit_idx, w.ccnt, w.ecnt);
+  hipLaunchKernelGGL((sel_compact_kernel<K>), grid, dim3(CMP_NT), 0, st, N, Ksel, chunk, nchunks,
+                     ordered ? 0 : 1, S, (const K*)w.blk_key, (const int*)w.blk_idx,
+                     (const int*)w.emit_idx, (const int*)w.ccnt, (const int*)w.ecnt,
+                     (K*)w.cand_key, w.cand_idx, out);
   hipLaunchKernelGGL((sel_refine_kernel<K>), dim3(segs), dim3(1024), 0, st, N, Ksel,
                      ordered ? 0 : 1, S, (const K*)w.cand_key, (const int*)w.cand_idx, out);
   if (ordered) {
