@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget for the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=0x5FF70002)
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each transform from a hipGraph instead of enqueuing its "
+                         "kernels eagerly (measured equal at n=2^27: launches keep ahead)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB before the transforms "
                          "(a known-size kernel that calibrates HBM byte counters)")
@@ -98,41 +101,40 @@ def main():
     front = "gather" if args.front_end == "auto" else args.front_end
     plan = S.Plan(prm, args.precision, front_end=front)
     plan.set_filter(ft, win)
-    plan.set_profiling(True)
-
-    if args.copy_probe_mib > 0:
-        nb = args.copy_probe_mib << 20
-        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"[bench] copy probe: {nb} bytes read, {nb} bytes written")
-        del src, dst
+    if args.graph:
+        plan.set_graph(True)
 
     # ---- warm-up, then K timed transforms (Node[I_F] host-visible at the end of each) ----
     for _ in range(args.warmup):
         nodes, cnt = plan.execute(x, a, ai, offs, copy=False)
     torch.cuda.synchronize()
-    stage_sum = None
-    iv_sum = None
     step_ms = []
     t_all = time.perf_counter()
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        nodes, cnt = plan.execute(x, a, ai, offs, copy=False)
+        nodes, cnt = plan.execute(x, a, ai, offs, copy=False, sync_inputs=False)
         step_ms.append((time.perf_counter() - t0) * 1e3)
+    torch.cuda.synchronize()
+    total_s = time.perf_counter() - t_all
+    ms_per_step = total_s * 1e3 / args.steps
+
+    # ---- stage breakdown: the same transform run eagerly with hipEvents at the stage
+    # boundaries (the graph replays the same kernels) ----
+    plan.set_profiling(True)
+    prof_steps = max(3, min(args.steps, 10))
+    stage_sum = iv_sum = None
+    for _ in range(prof_steps):
+        plan.execute(x, a, ai, offs, copy=False, sync_inputs=False)
         st = plan.stage_times()
         stage_sum = st if stage_sum is None else {kk: stage_sum[kk] + v for kk, v in st.items()}
         iv = plan.interval_times()
         iv_sum = iv if iv_sum is None else {kk: iv_sum[kk] + v for kk, v in iv.items()}
-    torch.cuda.synchronize()
-    total_s = time.perf_counter() - t_all
-    ms_per_step = total_s * 1e3 / args.steps
-    stages_ms = {kk: v * 1e3 / args.steps for kk, v in stage_sum.items()}
-    iv_ms = {kk: v * 1e3 / args.steps for kk, v in iv_sum.items()}
+    plan.set_profiling(False)
+    stages_ms = {kk: v * 1e3 / prof_steps for kk, v in stage_sum.items()}
+    iv_ms = {kk: v * 1e3 / prof_steps for kk, v in iv_sum.items()}
     log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (median "
-        f"{float(np.median(step_ms)):.3f}); stages(ms) "
+        f"{float(np.median(step_ms)):.3f}, {'graph' if args.graph else 'eager'}); "
+        f"stages(ms, eager + events) "
         + " ".join(f"{kk}={v:.3f}" for kk, v in stages_ms.items()))
 
     acc = S.metric(nodes, k, locs, X)
@@ -201,6 +203,7 @@ def main():
         "vs_baseline": round(ms_per_step / PUBLISHED_MS, 6) if args.config == "cfg2" and
         not fp32 else None,
         "dtype": "f32" if fp32 else "f64", "data": "synthetic", "front_end": front,
+        "launch": "hipGraph" if args.graph else "eager",
         "config": {"workload": f"{args.config}: sfft n=2^{q} k={k} Bcst={Bcst} "
                                f"loops={ll}+{le} thresh={lt} RLFC={rc}x{rl} "
                                f"{'noise ' + str(snr) + 'dB' if snr else 'noiseless'}, "

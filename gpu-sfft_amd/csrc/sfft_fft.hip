@@ -78,12 +78,18 @@ constexpr int FFT_NT = 512;
 constexpr int FFT_QMAX = 16;                 // columns per tile
 // complex elements per tile (LDS): 8192 complex64 (66 KB) or 4096 complex128 (66 KB), so
 // that two workgroups share a CU and one loads while the other computes
-template <typename Real>
+// The pass that gathers the comb samples from the signal (random lines) uses one 2048-point
+// column per tile (32 KB FP64 / 16 KB FP32), so four workgroups share a CU and more gathers
+// are in flight while other workgroups compute.
+template <typename Real, int LOAD = 0>
 struct Tile {
-  static constexpr int cap = sizeof(Real) == 8 ? 4096 : 8192;
+  static constexpr int cap = LOAD == 1 ? 2048 : (sizeof(Real) == 8 ? 4096 : 8192);
   static constexpr int ept = cap / FFT_NT;   // elements per thread
 };
-static int tile_cap(int is_fp32) { return is_fp32 ? Tile<float>::cap : Tile<double>::cap; }
+static int tile_cap(int is_fp32, int load) {
+  if (load == 1) return Tile<float, 1>::cap;
+  return is_fp32 ? Tile<float>::cap : Tile<double>::cap;
+}
 
 // Workgroups are dealt to the 8 XCDs round-robin by launch order.  Tiles t, t+1, ... read
 // neighbouring columns of the same 128-byte lines, so give the workgroups of one XCD (ids
@@ -216,7 +222,7 @@ template <typename Real, int LOAD, int STORE>
 __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
   using T2 = typename CT<Real>::T2;
   using KT = typename CT<Real>::K;
-  constexpr int CAP = Tile<Real>::cap, EPT = Tile<Real>::ept;
+  constexpr int CAP = Tile<Real, LOAD>::cap, EPT = Tile<Real, LOAD>::ept;
   __shared__ T2 lds[CAP + CAP / 16 + FFT_QMAX];
   const int tid = threadIdx.x;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -356,7 +362,8 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.log2Lprev = log2Lprev;
     // columns per tile: as many as fit the tile, up to qmax (does not change the arithmetic)
     const int qmax = (L.qmax > 0 && L.qmax <= FFT_QMAX) ? L.qmax : FFT_QMAX;
-    const int cap = tile_cap(T.is_fp32);
+    const int load = (p == 0) ? L.load : kLoadPlain;
+    const int cap = tile_cap(T.is_fp32, load);
     int log2Q = 0;
     while ((1 << (log2Q + 1)) <= qmax && ((1 << (log2Q + 1)) << r) <= cap) ++log2Q;
     int tiles;
@@ -378,7 +385,6 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.comb_x = L.comb_x;
     A.comb_offs = L.comb_offs;
     A.comb_sigma = L.comb_sigma;
-    const int load = (p == 0) ? L.load : kLoadPlain;
     const int store = last ? L.store : kStoreCplx;
     hipError_t e = T.is_fp32 ? dispatch_pass<float>(A, load, store, tiles, L.batch, st)
                              : dispatch_pass<double>(A, load, store, tiles, L.batch, st);

@@ -84,14 +84,23 @@ hipError_t hash_clear_launch(const HashArgs& h, hipStream_t st) {
   return fill_u32(h.maxfirst, 0xFFFFFFFFu, (size_t)h.loops_loc, st);
 }
 
+// approved residue first = J2 * ai mod W of every comb location, as a bitmap per location
+// loop, and the largest one (the max is reduced across the wave first: one atomic per wave on
+// the loop's single max word instead of one per residue)
 __global__ void hash_bitmap_kernel(HashArgs h) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int l = blockIdx.y;
-  if (i >= h.nJ2) return;
-  const uint64_t j2 = (uint64_t)h.J2[i];
-  const uint32_t first = (uint32_t)((j2 * (uint64_t)(uint32_t)h.ai_dev[l]) & (uint64_t)(h.W - 1));
-  atomicOr(&h.bitmap[(size_t)l * h.bm_words + (first >> 5)], 1u << (first & 31));
-  atomicMax(&h.maxfirst[l], (int)first);
+  int v = -1;
+  if (i < h.nJ2) {
+    const uint64_t j2 = (uint64_t)h.J2[i];
+    const uint32_t first =
+        (uint32_t)((j2 * (uint64_t)(uint32_t)h.ai_dev[l]) & (uint64_t)(h.W - 1));
+    atomicOr(&h.bitmap[(size_t)l * h.bm_words + (first >> 5)], 1u << (first & 31));
+    v = (int)first;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0 && v >= 0) atomicMax(&h.maxfirst[l], v);
 }
 
 hipError_t hash_bitmap_launch(const HashArgs& h, hipStream_t st) {

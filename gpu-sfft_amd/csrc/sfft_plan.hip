@@ -314,10 +314,14 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   return SFFT_OK;
 }
 
+static void drop_plan_graph(sfft_plan* P);
+
 extern "C" void sfft_plan_destroy(sfft_plan* P) {
   if (!P) return;
   if (P->st) (void)hipStreamSynchronize(P->st);
   else (void)hipDeviceSynchronize();
+  drop_plan_graph(P);
+  if (P->own_st) (void)hipStreamDestroy(P->st);
   for (auto& e : P->ev)
     if (e) (void)hipEventDestroy(e);
   if (P->up_ev) (void)hipEventDestroy(P->up_ev);
@@ -486,6 +490,13 @@ sfft_status run_back(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t
   return SFFT_OK;
 }
 
+static void drop_plan_graph(sfft_plan* P) {
+  if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  P->gexec = nullptr;
+  P->graph = nullptr;
+}
+
 extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const sfft_draws* dr,
                                           Node* d_nodes, int* d_count) {
   if (!P || !d_x) return SFFT_ERR_INVALID;
@@ -493,9 +504,51 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   sfft_status v = upload_draws(P, dr);
   if (v != SFFT_OK) return v;
   const HashArgs h = hash_args(P, d_count);
-  if ((v = run_front(P, d_x, h, P->st, true)) != SFFT_OK) return v;
-  if ((v = run_fold(P, d_x, P->st)) != SFFT_OK) return v;
-  return run_back(P, h, d_nodes, P->st, true);
+  if (!P->use_graph || P->profile) {   // eager (the profiling events need it)
+    if ((v = run_front(P, d_x, h, P->st, true)) != SFFT_OK) return v;
+    if ((v = run_fold(P, d_x, P->st)) != SFFT_OK) return v;
+    return run_back(P, h, d_nodes, P->st, true);
+  }
+  if (P->gexec && (P->g_x != d_x || P->g_nodes != d_nodes || P->g_count != d_count))
+    drop_plan_graph(P);
+  if (!P->gexec) {
+    HIPCK(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
+    v = run_front(P, d_x, h, P->st, false);
+    if (v == SFFT_OK) v = run_fold(P, d_x, P->st);
+    if (v == SFFT_OK) v = run_back(P, h, d_nodes, P->st, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(P->st, &g);
+    if (v != SFFT_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return v != SFFT_OK ? v : SFFT_ERR_HIP;
+    }
+    P->graph = g;
+    if (hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+      drop_plan_graph(P);
+      return SFFT_ERR_HIP;
+    }
+    P->g_x = d_x;
+    P->g_nodes = d_nodes;
+    P->g_count = d_count;
+  }
+  HIPCK(hipGraphLaunch(P->gexec, P->st));
+  return SFFT_OK;
+}
+
+// Graph mode: the transform's launch sequence is captured once and replayed (the draws are
+// uploaded before each replay, outside the graph).  A plan created without a stream gets a
+// stream of its own for this (the legacy default stream cannot be captured).
+extern "C" sfft_status sfft_plan_set_graph(sfft_plan* P, int on) {
+  if (!P) return SFFT_ERR_INVALID;
+  drop_plan_graph(P);
+  if (on && !P->st) {
+    HIPCK(hipStreamSynchronize(nullptr));
+    if (P->up_pending) HIPCK(hipEventSynchronize(P->up_ev));
+    HIPCK(hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking));
+    P->own_st = 1;
+  }
+  P->use_graph = on ? 1 : 0;
+  return SFFT_OK;
 }
 
 // upload one transform's draws (after the previous upload out of the staging has completed)

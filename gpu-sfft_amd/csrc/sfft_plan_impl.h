@@ -57,6 +57,15 @@ struct sfft_plan {
   int up_pending = 0;
   // optional stage timing (hipEvents at the reference's stage boundaries)
   int profile = 0;
+  // optional hipGraph of the whole transform (front, fold, back), replayed while the signal
+  // and output pointers stay the same; own_st: a stream made for it when the plan had none
+  int use_graph = 0;
+  int own_st = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const void* g_x = nullptr;
+  Node* g_nodes = nullptr;
+  int* g_count = nullptr;
   hipEvent_t ev[9] = {nullptr, nullptr, nullptr, nullptr, nullptr,
                       nullptr, nullptr, nullptr, nullptr};
 };

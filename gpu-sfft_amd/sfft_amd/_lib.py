@@ -59,7 +59,8 @@ EXPORTED_FUNCTIONS = [
     "run_experiment", "cuShift", "sfft_srandom", "sfft_random",
     "sfft_status_string", "sfft_params_validate", "sfft_plan_create", "sfft_plan_destroy",
     "sfft_plan_workspace_bytes", "sfft_plan_set_filter", "sfft_execute_async", "sfft_execute",
-    "sfft_plan_set_profiling", "sfft_plan_stage_times", "sfft_plan_interval_times",
+    "sfft_plan_set_profiling", "sfft_plan_set_graph", "sfft_plan_stage_times",
+    "sfft_plan_interval_times",
     "sfft_plan_debug_buffer", "sfft_value_apply", "sfft_value_adjoint",
     "sfft_batch_create", "sfft_batch_destroy", "sfft_batch_plan", "sfft_batch_set_filter",
     "sfft_batch_set_graph", "sfft_batch_execute_async", "sfft_batch_execute",
@@ -109,6 +110,8 @@ def _declare(L):
     L.sfft_execute.restype = _i
     L.sfft_plan_set_profiling.argtypes = [_p, _i]
     L.sfft_plan_set_profiling.restype = _i
+    L.sfft_plan_set_graph.argtypes = [_p, _i]
+    L.sfft_plan_set_graph.restype = _i
     L.sfft_plan_stage_times.argtypes = [_p, _p]
     L.sfft_plan_stage_times.restype = _i
     L.sfft_plan_interval_times.argtypes = [_p, _p]

@@ -162,6 +162,10 @@ class Plan:
     def set_profiling(self, on=True):
         _check(lib.sfft_plan_set_profiling(self._h, int(on)), "sfft_plan_set_profiling")
 
+    def set_graph(self, on=True):
+        """Capture the transform into a hipGraph on first use and replay it afterwards."""
+        _check(lib.sfft_plan_set_graph(self._h, int(on)), "sfft_plan_set_graph")
+
     def stage_times(self):
         out = np.zeros(7, dtype=np.float64)
         _check(lib.sfft_plan_stage_times(self._h, _np_ptr(out)), "sfft_plan_stage_times")
@@ -189,11 +193,14 @@ class Plan:
                 not x.is_contiguous():
             raise TypeError(f"x must be a contiguous {want} device tensor of n entries")
 
-    def execute(self, x, a, ai, offsets, copy=True):
+    def execute(self, x, a, ai, offsets, copy=True, sync_inputs=True):
         """Runs one transform and returns (nodes[I_F] numpy structured array, I_F).  With
         copy=False the nodes are a view of the plan's pinned buffer (valid until the next
-        execute)."""
+        execute).  sync_inputs: wait for pending device work first (the plan may run on a
+        stream of its own); False when x is known to be ready."""
         self._check_x(x)
+        if sync_inputs:
+            _torch().cuda.synchronize()
         dr = self._draws(a, ai, offsets)
         cnt = C.c_int()
         st = lib.sfft_execute(self._h, _ptr(x), C.byref(dr), _np_ptr(self._nodes),

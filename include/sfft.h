@@ -192,6 +192,13 @@ sfft_status sfft_execute_async(sfft_plan *plan, const void *d_x, const sfft_draw
 sfft_status sfft_execute(sfft_plan *plan, const void *d_x, const sfft_draws *draws,
                          Node *h_nodes, int *h_count);
 
+/* Graph mode (off by default): the transform's kernel sequence is captured into a hipGraph
+ * on first use and replayed by later sfft_execute[_async] calls while the signal and result
+ * pointers stay the same (re-captured when they change); the draws are uploaded before each
+ * replay.  A plan created with stream NULL gets a stream of its own for this.  Profiling
+ * (sfft_plan_set_profiling) runs the transform eagerly. */
+sfft_status sfft_plan_set_graph(sfft_plan *plan, int on);
+
 /* Stage timing with hipEvents on the plan stream (off by default).  After a completed
  * transform, sfft_plan_stage_times writes seconds in the reference's table order
  * (outerLoop.cu:174-187): votetable, restr-loc-index, perm+filter, reverse-hash,

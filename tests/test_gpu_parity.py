@@ -286,3 +286,31 @@ def test_transform_fp32_bitexact(S, cfg, front_end):
     assert cnt == ref["I_F"]
     assert np.array_equal(plan.debug_array(3, cnt, np.int32), ref["hits"])
     assert nodes.tobytes() == ref["nodes"].tobytes()
+
+
+def test_plan_graph_replays_match_eager(S):
+    """Graph mode: the captured transform, replayed with new draws, new signal contents behind
+    the same pointer and other device work in between, equals the eager transform bit for
+    bit on every replay (FP64 and FP32)."""
+    import torch
+    for prec in ("fp64", "fp32"):
+        prm = params(SMALL_CONFIGS[2])
+        n = prm["n"]
+        x, _, _ = sparse_signal(n, prm["k"], 5)
+        cdt = np.complex64 if prec == "fp32" else np.complex128
+        ft, ff = oracle_filter(prm)
+        win = O.filter_window_slice(ff, n, prm["B"])
+        eager = S.Plan(prm, prec)
+        eager.set_filter(_dev(ft.astype(cdt)), win)
+        graph = S.Plan(prm, prec)
+        graph.set_filter(_dev(ft.astype(cdt)), win)
+        graph.set_graph(True)
+        dx = _dev(x.astype(cdt))
+        for rep in range(4):
+            a, ai, offs = draws(n, prm["loops"], prm["RLFC_loops"], prm["W_RLFC"], 40 + rep)
+            want = eager.execute(dx, a, ai, offs)
+            got = graph.execute(dx, a, ai, offs)
+            assert got[1] == want[1] and got[0].tobytes() == want[0].tobytes(), (prec, rep)
+            y = torch.randn(1 << 18, device="cuda")
+            (y * 3).sum().item()
+            dx.mul_(0.75).add_(torch.roll(dx, 1) * 0.5)
