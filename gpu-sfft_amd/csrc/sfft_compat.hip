@@ -75,6 +75,18 @@ extern "C" long sfft_random(void) {
   return (long)r;
 }
 
+// outer_loop's plan and pinned node buffer, kept between calls (released with the process)
+namespace {
+struct OuterCache {
+  sfft_plan* plan = nullptr;
+  sfft_params p{};
+  int device = -1;
+  Node* nodes = nullptr;
+};
+OuterCache g_outer;
+std::mutex g_outer_mu;
+}  // namespace
+
 static int gcd_(int a, int b) {   // utils.c:81-84
   while (b) {
     const int r = a % b;
@@ -142,12 +154,34 @@ extern "C" Node* outer_loop(complex_t* d_x, int n, complex_t* d_filter_time,
     a[l] = av;
     ai[l] = mod_inverse_(av, n);
   }
-  sfft_plan* plan = nullptr;
-  sfft_status s = sfft_plan_create(&p, SFFT_FP64, nullptr, &plan);
-  if (s != SFFT_OK) {
-    fprintf(stderr, "outer_loop: %s\n", sfft_status_string(s));
+  // the plan (workspace, FFT tables) and a pinned node buffer are kept from one call to the
+  // next while the parameters and the device stay the same (callers transform repeatedly)
+  std::lock_guard<std::mutex> lk(g_outer_mu);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (g_outer.plan && (g_outer.device != dev || memcmp(&g_outer.p, &p, sizeof p) != 0)) {
+    sfft_plan_destroy(g_outer.plan);
+    g_outer.plan = nullptr;
+  }
+  sfft_status s = SFFT_OK;
+  if (!g_outer.plan) {
+    s = sfft_plan_create(&p, SFFT_FP64, nullptr, &g_outer.plan);
+    if (s != SFFT_OK) {
+      g_outer.plan = nullptr;
+      fprintf(stderr, "outer_loop: %s\n", sfft_status_string(s));
+      return nullptr;
+    }
+    g_outer.p = p;
+    g_outer.device = dev;
+  }
+  if (!g_outer.nodes &&
+      hipHostMalloc((void**)&g_outer.nodes, sizeof(Node) * MAXSIZE_HITS, hipHostMallocDefault) !=
+          hipSuccess) {
+    g_outer.nodes = nullptr;
+    fprintf(stderr, "outer_loop: cannot allocate the pinned node buffer\n");
     return nullptr;
   }
+  sfft_plan* plan = g_outer.plan;
   const int m = n / B;
   std::vector<complex_t> win(m), seg(m);
   // the m entries of filter_freq around 0 (host or device pointer)
@@ -157,25 +191,23 @@ extern "C" Node* outer_loop(complex_t* d_x, int n, complex_t* d_filter_time,
                             (size_t)(m - m / 2) * sizeof(complex_t), hipMemcpyDefault);
   if (e1 != hipSuccess || e2 != hipSuccess) {
     fprintf(stderr, "outer_loop: cannot read filter_freq\n");
-    sfft_plan_destroy(plan);
     return nullptr;
   }
   sfft_plan_set_filter(plan, d_filter_time, win.data());
-  if (TIMING) sfft_plan_set_profiling(plan, 1);
+  sfft_plan_set_profiling(plan, TIMING ? 1 : 0);
   sfft_draws dr{a.data(), ai.data(), offs.data()};
-  std::vector<Node> nodes(MAXSIZE_HITS);
+  Node* nodes = g_outer.nodes;
   int count = 0;
-  s = sfft_execute(plan, d_x, &dr, nodes.data(), &count);
+  s = sfft_execute(plan, d_x, &dr, nodes, &count);
   double st7[7] = {0, 0, 0, 0, 0, 0, 0};
   if (TIMING) sfft_plan_stage_times(plan, st7);
-  sfft_plan_destroy(plan);
   if (s != SFFT_OK) {
     fprintf(stderr, "outer_loop: %s (%d hits)\n", sfft_status_string(s), count);
     return nullptr;
   }
   *I_F = count;
   Node* out = (Node*)malloc(sizeof(Node) * (size_t)(count > 0 ? count : 1));
-  memcpy(out, nodes.data(), sizeof(Node) * (size_t)count);
+  memcpy(out, nodes, sizeof(Node) * (size_t)count);
   const double DDD = wall_now() - t_begin;
   if (TIMING) {   // outerLoop.cu:174-187
     const double oth = DDD - st7[0] - st7[1] - st7[2] - st7[3] - st7[4] - st7[5];
