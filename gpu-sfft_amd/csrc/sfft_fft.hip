@@ -117,6 +117,7 @@ struct PassArgs {
   const void* comb_x;
   const int* comb_offs;
   int comb_sigma;
+  int nbatch;
 };
 
 template <typename T2>
@@ -225,8 +226,19 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
   constexpr int CAP = Tile<Real, LOAD>::cap, EPT = Tile<Real, LOAD>::ept;
   __shared__ T2 lds[CAP + CAP / 16 + FFT_QMAX];
   const int tid = threadIdx.x;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const long long b = blockIdx.y;
+  // The comb-gathering pass runs all comb loops in one grid dimension, loop index fastest,
+  // so the same tile of every loop is handled back to back on one XCD: comb loops whose
+  // offsets fall in the same 128-byte window read the same lines, fetched once.
+  int tile;
+  long long b;
+  if (LOAD == kLoadComb) {
+    const int s = xcd_tile(blockIdx.x, gridDim.x);
+    b = s % A.nbatch;
+    tile = s / A.nbatch;
+  } else {
+    tile = xcd_tile(blockIdx.x, gridDim.x);
+    b = blockIdx.y;
+  }
   const int r = A.r;
   const int R = 1 << r;
   const int RP = R + (R >> 4) + 1;   // padded column stride (see LPAD)
@@ -312,7 +324,8 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
 
 template <typename Real, int LOAD, int STORE>
 static hipError_t launch_pass(const PassArgs& A, int tiles, int batch, hipStream_t st) {
-  hipLaunchKernelGGL((fft_pass_kernel<Real, LOAD, STORE>), dim3(tiles, batch), dim3(FFT_NT), 0,
+  const dim3 grid = (LOAD == kLoadComb) ? dim3(tiles * batch, 1) : dim3(tiles, batch);
+  hipLaunchKernelGGL((fft_pass_kernel<Real, LOAD, STORE>), grid, dim3(FFT_NT), 0,
                      st, A);
   return hipGetLastError();
 }
@@ -385,6 +398,7 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.comb_x = L.comb_x;
     A.comb_offs = L.comb_offs;
     A.comb_sigma = L.comb_sigma;
+    A.nbatch = L.batch;
     const int store = last ? L.store : kStoreCplx;
     hipError_t e = T.is_fp32 ? dispatch_pass<float>(A, load, store, tiles, L.batch, st)
                              : dispatch_pass<double>(A, load, store, tiles, L.batch, st);

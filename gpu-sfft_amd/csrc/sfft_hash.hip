@@ -85,10 +85,12 @@ hipError_t hash_clear_launch(const HashArgs& h, hipStream_t st) {
 }
 
 // approved residue first = J2 * ai mod W of every comb location, as a bitmap per location
-// loop, and the largest one (the max is reduced across the wave first: one atomic per wave on
-// the loop's single max word instead of one per residue)
-__global__ void hash_bitmap_kernel(HashArgs h) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// loop, and the largest one (reduced across the block first: one atomic per block on the
+// loop's single max word -- same-address atomics serialise)
+constexpr int BM_NT = 1024;
+__global__ __launch_bounds__(BM_NT) void hash_bitmap_kernel(HashArgs h) {
+  __shared__ int wmax[BM_NT / 64];
+  const int i = blockIdx.x * BM_NT + threadIdx.x;
   const int l = blockIdx.y;
   int v = -1;
   if (i < h.nJ2) {
@@ -100,12 +102,18 @@ __global__ void hash_bitmap_kernel(HashArgs h) {
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  if ((threadIdx.x & 63) == 0 && v >= 0) atomicMax(&h.maxfirst[l], v);
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = -1;
+    for (int w = 0; w < BM_NT / 64; ++w) m = max(m, wmax[w]);
+    if (m >= 0) atomicMax(&h.maxfirst[l], m);
+  }
 }
 
 hipError_t hash_bitmap_launch(const HashArgs& h, hipStream_t st) {
-  dim3 grid((h.nJ2 + 255) / 256, h.loops_loc);
-  hipLaunchKernelGGL(hash_bitmap_kernel, grid, dim3(256), 0, st, h);
+  dim3 grid((h.nJ2 + BM_NT - 1) / BM_NT, h.loops_loc);
+  hipLaunchKernelGGL(hash_bitmap_kernel, grid, dim3(BM_NT), 0, st, h);
   return hipGetLastError();
 }
 

@@ -306,7 +306,10 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
       hipHostMalloc((void**)&P->h_draws, (size_t)(2 * L + Rl) * 4, hipHostMallocDefault) !=
           hipSuccess ||
       hipHostMalloc((void**)&P->h_count, 16, hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&P->up_ev, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&P->up_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&P->st2, hipStreamNonBlocking) != hipSuccess) {
     sfft_plan_destroy(P);
     return SFFT_ERR_ALLOC;
   }
@@ -320,8 +323,12 @@ extern "C" void sfft_plan_destroy(sfft_plan* P) {
   if (!P) return;
   if (P->st) (void)hipStreamSynchronize(P->st);
   else (void)hipDeviceSynchronize();
+  if (P->st2) (void)hipStreamSynchronize(P->st2);
   drop_plan_graph(P);
   if (P->own_st) (void)hipStreamDestroy(P->st);
+  if (P->st2) (void)hipStreamDestroy(P->st2);
+  if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
+  if (P->ev_join) (void)hipEventDestroy(P->ev_join);
   for (auto& e : P->ev)
     if (e) (void)hipEventDestroy(e);
   if (P->up_ev) (void)hipEventDestroy(P->up_ev);
@@ -452,16 +459,12 @@ sfft_status run_fold(sfft_plan* P, const void* d_x, hipStream_t st) {
   return SFFT_OK;
 }
 
-// back: bucket FFT + cutoff, reverse hash, estimation
-sfft_status run_back(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t st,
-                            bool marks) {
+// bucket: bucket FFT + cutoff (after the fold)
+sfft_status run_bucket(sfft_plan* P, hipStream_t st, bool marks) {
   const sfft_params& p = P->p;
   const int L = p.loops, Ll = p.loops_loc, K = p.B_thresh;
   const long long B = p.B;
-  auto mark = [&](int i) -> hipError_t {
-    return (marks && P->profile) ? hipEventRecord(P->ev[i], st) : hipSuccess;
-  };
-  HIPCK(mark(4));
+  if (marks && P->profile) HIPCK(hipEventRecord(P->ev[4], st));
   FftLaunch f;
   f.tab = &P->tabB;
   f.in = P->bins_t;
@@ -478,16 +481,49 @@ sfft_status run_back(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t
   HIPCK(fft_run(f, st));
   // ascending bucket lists: the hit list is ordered by (loop, bucket, walk)
   HIPCK(select_topk(P->fp32, P->bkeys, B, (int)B, K, Ll, P->J, 1, P->selB, st));
-  HIPCK(mark(5));
-  // reverse hash: votes, then the hit list in (loop, bucket, walk) order
+  if (marks && P->profile) HIPCK(hipEventRecord(P->ev[5], st));
+  return SFFT_OK;
+}
+
+// locate: reverse hash (needs the comb bitmaps and the bucket cutoffs) + estimation
+sfft_status run_locate(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t st,
+                       bool marks) {
+  const sfft_params& p = P->p;
   HIPCK(hash_vote_launch(h, st));
   HIPCK(hash_hits_launch(h, st));
-  HIPCK(mark(6));
-  // estimation
-  HIPCK(estimate_launch(P->fp32, P->hits, h.count, P->cap, P->bins_f, L, p.n, P->d_ai, p.B,
-                        P->win, d_nodes ? d_nodes : P->nodes, P->sel, st));
-  HIPCK(mark(7));
+  if (marks && P->profile) HIPCK(hipEventRecord(P->ev[6], st));
+  HIPCK(estimate_launch(P->fp32, P->hits, h.count, P->cap, P->bins_f, p.loops, p.n, P->d_ai,
+                        p.B, P->win, d_nodes ? d_nodes : P->nodes, P->sel, st));
+  if (marks && P->profile) HIPCK(hipEventRecord(P->ev[7], st));
   return SFFT_OK;
+}
+
+sfft_status run_back(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t st,
+                     bool marks) {
+  sfft_status v = run_bucket(P, st, marks);
+  return v != SFFT_OK ? v : run_locate(P, h, d_nodes, st, marks);
+}
+
+// One transform as two concurrent chains joined before the reverse hash: the comb chain
+// (clear, comb FFT, cutoff, bitmaps) on the plan stream and the bucket chain (permute+filter,
+// bucket FFT, cutoff) on a second stream, so the latency-bound kernels of one chain fill the
+// gaps of the other's gathers.  Profiling (and the streaming front end, whose pass feeds the
+// fold) runs everything serially on the plan stream.
+static sfft_status run_all(sfft_plan* P, const void* d_x, const HashArgs& h, Node* d_nodes) {
+  sfft_status v;
+  if (P->profile || P->stream) {   // (the streaming pass feeds the fold: one chain)
+    if ((v = run_front(P, d_x, h, P->st, true)) != SFFT_OK) return v;
+    if ((v = run_fold(P, d_x, P->st)) != SFFT_OK) return v;
+    return run_back(P, h, d_nodes, P->st, true);
+  }
+  HIPCK(hipEventRecord(P->ev_fork, P->st));
+  HIPCK(hipStreamWaitEvent(P->st2, P->ev_fork, 0));
+  if ((v = run_fold(P, d_x, P->st2)) != SFFT_OK) return v;
+  if ((v = run_bucket(P, P->st2, false)) != SFFT_OK) return v;
+  HIPCK(hipEventRecord(P->ev_join, P->st2));
+  if ((v = run_front(P, d_x, h, P->st, false)) != SFFT_OK) return v;
+  HIPCK(hipStreamWaitEvent(P->st, P->ev_join, 0));
+  return run_locate(P, h, d_nodes, P->st, false);
 }
 
 static void drop_plan_graph(sfft_plan* P) {
@@ -504,18 +540,12 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
   sfft_status v = upload_draws(P, dr);
   if (v != SFFT_OK) return v;
   const HashArgs h = hash_args(P, d_count);
-  if (!P->use_graph || P->profile) {   // eager (the profiling events need it)
-    if ((v = run_front(P, d_x, h, P->st, true)) != SFFT_OK) return v;
-    if ((v = run_fold(P, d_x, P->st)) != SFFT_OK) return v;
-    return run_back(P, h, d_nodes, P->st, true);
-  }
+  if (!P->use_graph || P->profile) return run_all(P, d_x, h, d_nodes);   // eager
   if (P->gexec && (P->g_x != d_x || P->g_nodes != d_nodes || P->g_count != d_count))
     drop_plan_graph(P);
   if (!P->gexec) {
     HIPCK(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
-    v = run_front(P, d_x, h, P->st, false);
-    if (v == SFFT_OK) v = run_fold(P, d_x, P->st);
-    if (v == SFFT_OK) v = run_back(P, h, d_nodes, P->st, false);
+    v = run_all(P, d_x, h, d_nodes);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(P->st, &g);
     if (v != SFFT_OK || e != hipSuccess) {

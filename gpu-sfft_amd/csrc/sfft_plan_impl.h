@@ -10,6 +10,8 @@ struct sfft_plan {
   sfft_params p;
   int fp32 = 0;
   hipStream_t st = nullptr;
+  hipStream_t st2 = nullptr;    // second chain (permute+filter, bucket FFT, cutoff)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int log2n = 0, log2B = 0, log2W = 0, m = 0;
   int bpe = 1;
   int cap = MAXSIZE_HITS;
@@ -86,3 +88,6 @@ sfft_status run_front(sfft_plan* P, const void* d_x, const HashArgs& h, hipStrea
                       bool marks);
 sfft_status run_fold(sfft_plan* P, const void* d_x, hipStream_t st);
 sfft_status run_back(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t st, bool marks);
+sfft_status run_bucket(sfft_plan* P, hipStream_t st, bool marks);
+sfft_status run_locate(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t st,
+                       bool marks);
