@@ -708,15 +708,22 @@ extern "C" sfft_status sfft_execute(sfft_plan* P, const void* d_x, const sfft_dr
   if (!P) return SFFT_ERR_INVALID;
   sfft_status s = sfft_execute_async(P, d_x, dr, P->nodes, P->count);
   if (s != SFFT_OK) return s;
+  // the count and, speculatively, as many nodes as the previous transform had plus 1/8 go
+  // back in one round trip; the rest (if the count grew past that) in a second one
   HIPCK(hipMemcpyAsync(P->h_count, P->count, 4, hipMemcpyDeviceToHost, P->st));
+  long long guess = (long long)P->last_count + P->last_count / 8;
+  if (guess > P->cap) guess = P->cap;
+  if (h_nodes && guess > 0)
+    HIPCK(hipMemcpyAsync(h_nodes, P->nodes, (size_t)guess * sizeof(Node), hipMemcpyDeviceToHost,
+                         P->st));
   HIPCK(hipStreamSynchronize(P->st));
   const int cnt = *P->h_count;
   P->last_count = cnt;
   if (h_count) *h_count = cnt;
-  const int ncopy = cnt < P->cap ? cnt : P->cap;
-  if (h_nodes && ncopy > 0) {
-    HIPCK(hipMemcpyAsync(h_nodes, P->nodes, (size_t)ncopy * sizeof(Node), hipMemcpyDeviceToHost,
-                         P->st));
+  const long long ncopy = cnt < P->cap ? cnt : P->cap;
+  if (h_nodes && ncopy > guess) {
+    HIPCK(hipMemcpyAsync(h_nodes + guess, P->nodes + guess, (size_t)(ncopy - guess) * sizeof(Node),
+                         hipMemcpyDeviceToHost, P->st));
     HIPCK(hipStreamSynchronize(P->st));
   }
   return cnt > P->cap ? SFFT_ERR_HIT_OVERFLOW : SFFT_OK;
