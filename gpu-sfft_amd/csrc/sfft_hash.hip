@@ -169,6 +169,16 @@ int hash_blocks(int loops_loc, int K, long long m) {
   return (int)(((long long)loops_loc * K * hash_lpj(m) + HASH_NT - 1) / HASH_NT);
 }
 
+// A location is voted at most once per loop (the J windows of a loop are disjoint), so its
+// vote count after loop l is the number of loops 0..l that walked it.  Hence (1) only loops
+// l >= loops_thresh-1 can produce hits, and (2) the hit test of loop l needs only the votes of
+// loops < l (its own is implied by the walk): the last location loop's votes are never read.
+// The vote pass covers loops [0, loops_loc-1), the count and write passes [thresh-1, loops_loc)
+// (loops_thresh >= 1 is checked when the plan is made).
+static int hash_vote_first(const HashArgs& h) { return 0; }
+static int hash_vote_count(const HashArgs& h) { return h.loops_loc - 1; }
+static int hash_hit_count(const HashArgs& h) { return h.loops_loc - (h.loops_thresh - 1); }
+
 // thread -> (location loop l, bucket index i, slice sub of lpj)
 struct HashSlot {
   int l, i, sub, lpj;
@@ -176,14 +186,21 @@ struct HashSlot {
   bool valid;
 };
 
-__device__ __forceinline__ HashSlot hash_slot(const HashArgs& h) {
+__device__ __forceinline__ int hash_hit_first_d(const HashArgs& h) { return h.loops_thresh - 1; }
+__device__ __forceinline__ int hash_hit_count_d(const HashArgs& h) {
+  return h.loops_loc - (h.loops_thresh - 1);
+}
+
+// threads cover loops [l_first, l_first + nloops)
+__device__ __forceinline__ HashSlot hash_slot(const HashArgs& h, int l_first, int nloops) {
   HashSlot s;
   const long long m = (long long)h.n / h.B;
   s.lpj = hash_lpj(m);
   const long long gid = (long long)blockIdx.x * HASH_NT + threadIdx.x;
-  s.jidx = gid / s.lpj;
+  const long long local = gid / s.lpj;
   s.sub = (int)(gid % s.lpj);
-  s.valid = s.jidx < (long long)h.loops_loc * h.K;
+  s.valid = local < (long long)nloops * h.K;
+  s.jidx = local + (long long)l_first * h.K;
   s.l = (int)(s.jidx / h.K);
   s.i = (int)(s.jidx - (long long)s.l * h.K);
   s.dl = m / s.lpj;
@@ -191,8 +208,9 @@ __device__ __forceinline__ HashSlot hash_slot(const HashArgs& h) {
   return s;
 }
 
-__global__ __launch_bounds__(HASH_NT) void hash_vote_kernel(HashArgs h) {
-  const HashSlot s = hash_slot(h);
+__global__ __launch_bounds__(HASH_NT) void hash_vote_kernel(HashArgs h, int l_first,
+                                                             int nloops) {
+  const HashSlot s = hash_slot(h, l_first, nloops);
   if (!s.valid) return;
   const int lgepw = 5 - __ffs(h.bpe) + 1;  // log2(32 / bpe)
   const uint32_t epw_mask = (1u << lgepw) - 1u;
@@ -202,19 +220,23 @@ __global__ __launch_bounds__(HASH_NT) void hash_vote_kernel(HashArgs h) {
 }
 
 hipError_t hash_vote_launch(const HashArgs& h, hipStream_t st) {
-  const int nb = hash_blocks(h.loops_loc, h.K, (long long)h.n / h.B);
-  hipLaunchKernelGGL(hash_vote_kernel, dim3(nb), dim3(HASH_NT), 0, st, h);
+  const int nl = hash_vote_count(h);
+  if (nl <= 0) return hipSuccess;
+  const int nb = hash_blocks(nl, h.K, (long long)h.n / h.B);
+  hipLaunchKernelGGL(hash_vote_kernel, dim3(nb), dim3(HASH_NT), 0, st, h, hash_vote_first(h),
+                     nl);
   return hipGetLastError();
 }
 
-// a location is a hit of loop l when it is a candidate of l and exactly loops_thresh of the
-// loops 0..l voted for it (the loop in which the reference's count reaches the threshold)
+// a location walked by loop l is a hit of loop l when exactly loops_thresh of the loops 0..l
+// voted for it, loop l included (the loop in which the reference's count reaches the
+// threshold): loops_thresh - 1 of the loops before l
 __device__ __forceinline__ bool hash_is_hit(const HashArgs& h, int l, uint32_t loc, int lgepw,
                                             uint32_t epw_mask) {
   const uint32_t word = h.vote[loc >> lgepw];
   const uint32_t mask = (word >> ((loc & epw_mask) * (uint32_t)h.bpe)) &
                         (uint32_t)((1ull << h.bpe) - 1ull);
-  return __popc(mask & ((2u << l) - 1u)) == h.loops_thresh;
+  return __popc(mask & ((1u << l) - 1u)) + 1 == h.loops_thresh;
 }
 
 __device__ __forceinline__ int hash_count_slice(const HashArgs& h, const HashSlot& s) {
@@ -229,7 +251,7 @@ __device__ __forceinline__ int hash_count_slice(const HashArgs& h, const HashSlo
 
 __global__ __launch_bounds__(HASH_NT) void hash_count_kernel(HashArgs h) {
   __shared__ int sh[16];
-  const HashSlot s = hash_slot(h);
+  const HashSlot s = hash_slot(h, hash_hit_first_d(h), hash_hit_count_d(h));
   const int c = hash_count_slice(h, s);
   int tot = c;   // total of the bucket's lanes (aligned groups of lpj lanes)
   for (int o = 1; o < s.lpj; o <<= 1) tot += __shfl_xor(tot, o, 64);
@@ -269,7 +291,7 @@ __global__ __launch_bounds__(1024) void hash_scan_kernel(HashArgs h, int nblocks
 }
 
 __global__ __launch_bounds__(HASH_NT) void hash_write_kernel(HashArgs h) {
-  const HashSlot s = hash_slot(h);
+  const HashSlot s = hash_slot(h, hash_hit_first_d(h), hash_hit_count_d(h));
   const int c = hash_count_slice(h, s);
   // exclusive scan over the bucket's lanes (in window order)
   const int lane = threadIdx.x & 63;
@@ -291,7 +313,7 @@ __global__ __launch_bounds__(HASH_NT) void hash_write_kernel(HashArgs h) {
 }
 
 hipError_t hash_hits_launch(const HashArgs& h, hipStream_t st) {
-  const int nb = hash_blocks(h.loops_loc, h.K, (long long)h.n / h.B);
+  const int nb = hash_blocks(hash_hit_count(h), h.K, (long long)h.n / h.B);
   hipLaunchKernelGGL(hash_count_kernel, dim3(nb), dim3(HASH_NT), 0, st, h);
   hipLaunchKernelGGL(hash_scan_kernel, dim3(1), dim3(1024), 0, st, h, nb);
   hipLaunchKernelGGL(hash_write_kernel, dim3(nb), dim3(HASH_NT), 0, st, h);
