@@ -237,8 +237,9 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   P->log2W = ilog2_exact(prm->W_RLFC);
   P->m = prm->n / prm->B;
   P->cap = prm->hit_capacity > 0 ? prm->hit_capacity : MAXSIZE_HITS;
+  // vote bits per location: one per location loop whose votes are read (all but the last)
   P->bpe = 1;
-  while (P->bpe < prm->loops_loc) P->bpe <<= 1;
+  while (P->bpe < prm->loops_loc - 1) P->bpe <<= 1;
   const int L = prm->loops, Ll = prm->loops_loc, Rl = prm->RLFC_loops, K = prm->B_thresh;
   const size_t W = prm->W_RLFC, B = prm->B;
   const size_t e = P->fp32 ? 8 : 16, ke = P->fp32 ? 4 : 8;
@@ -740,6 +741,7 @@ extern "C" const void* sfft_plan_debug_buffer(const sfft_plan* P, int which) {
     case 5: return P->nodes;
     case 6: return P->count;
     case 7: return P->sel;
+    case 8: return P->comb_keys;
   }
   return nullptr;
 }

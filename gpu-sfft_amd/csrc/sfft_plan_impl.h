@@ -46,7 +46,10 @@ struct sfft_plan {
   double2* win = nullptr;
   Node* nodes = nullptr;
   uint8_t* sel = nullptr;       // median-selected loops of the last transform (2 per hit)
-  int qmax_W = 0, qmax_B = 0;   // FFT columns per tile (0 = default), set by the autotuner
+  // FFT columns per tile (0 = as many as fit); the bucket FFTs default to 4 columns so that a
+  // loops x B batch spreads over enough workgroups to fill the device (the autotuner of the
+  // batch executor may change both)
+  int qmax_W = 0, qmax_B = 4;
   int* adj_head = nullptr;      // adjoint scratch: loops*B list heads, 2*cap links
   int* adj_next = nullptr;
   const void* d_ft = nullptr;

@@ -331,6 +331,10 @@ __global__ __launch_bounds__(CMP_NT) void sel_compact_kernel(int N, int Ksel, in
 }
 
 // ---- 3. refine over the candidates (one workgroup per segment) ----------------------------
+// The first level scans the candidate list; from then on only the survivors of the previous
+// level (the composite keys under the chosen prefix, usually a handful) are scanned, from LDS
+// (back to the full list if they outgrow RF_CAP).
+constexpr int RF_CAP = 2048;
 template <typename K>
 __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int emit,
                                                           SelState* __restrict__ st,
@@ -338,6 +342,8 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
                                                           const int* __restrict__ cidx,
                                                           int* __restrict__ out) {
   __shared__ unsigned h[SEL_BINS];
+  __shared__ u128 lst[2][RF_CAP];
+  __shared__ int s_len[2];
   __shared__ int sh[16];
   __shared__ int s_digit, s_above, s_exit;
   __shared__ int s_out;
@@ -353,6 +359,7 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
   int consumed = SEL_DIG;
   u128 T = 0;
   bool done = false;
+  int cur = -1;   // survivors: -1 = the candidate list, else lst[cur][0 .. s_len[cur])
   // at most ceil(TB / SEL_DIG) levels; the bound only guards against a broken invariant
   for (int level = 0; !done && level < (TB + SEL_DIG - 1) / SEL_DIG; ++level) {
     const int width = min(SEL_DIG, TB - consumed);
@@ -364,10 +371,15 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
       s_exit = 1;
     }
     __syncthreads();
-    for (int j = tid; j < nc; j += 1024) {
-      const u128 c = composite<K>(ck[j], (unsigned)ci[j], N, IB);
-      if ((c >> (shift + width)) == prefix)
-        atomicAdd(&h[(unsigned)(c >> shift) & ((1u << width) - 1u)], 1u);
+    const unsigned dmask = (1u << width) - 1u;
+    if (cur < 0) {
+      for (int j = tid; j < nc; j += 1024) {
+        const u128 c = composite<K>(ck[j], (unsigned)ci[j], N, IB);
+        if ((c >> (shift + width)) == prefix) atomicAdd(&h[(unsigned)(c >> shift) & dmask], 1u);
+      }
+    } else {
+      for (int j = tid; j < s_len[cur]; j += 1024)
+        atomicAdd(&h[(unsigned)(lst[cur][j] >> shift) & dmask], 1u);
     }
     __syncthreads();
     const int nb = 1 << width;
@@ -401,6 +413,30 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
     if (s_exit) {
       T = prefix << shift;
       done = true;
+    } else {
+      // keep the survivors: the keys whose bits above `shift` equal the new prefix
+      const int nxt = (cur < 0) ? 0 : 1 - cur;
+      if (tid == 0) s_len[nxt] = 0;
+      __syncthreads();
+      if (cur < 0) {
+        for (int j = tid; j < nc; j += 1024) {
+          const u128 c = composite<K>(ck[j], (unsigned)ci[j], N, IB);
+          if ((c >> shift) == prefix) {
+            const int p = atomicAdd(&s_len[nxt], 1);
+            if (p < RF_CAP) lst[nxt][p] = c;
+          }
+        }
+      } else {
+        for (int j = tid; j < s_len[cur]; j += 1024) {
+          const u128 c = lst[cur][j];
+          if ((c >> shift) == prefix) {
+            const int p = atomicAdd(&s_len[nxt], 1);
+            if (p < RF_CAP) lst[nxt][p] = c;
+          }
+        }
+      }
+      __syncthreads();
+      cur = (s_len[nxt] <= RF_CAP) ? nxt : -1;
     }
     __syncthreads();
   }

@@ -213,7 +213,8 @@ sfft_status sfft_plan_interval_times(sfft_plan *plan, double *seconds7);
  * 0 bins_f (loops x B complex), 1 J (loops_loc x B_thresh int), 2 J2 lists (RLFC_loops x
  * B_thresh int), 3 hits (int, count from the last execute), 4 bins_t (loops x B complex),
  * 5 nodes (Node[hit_capacity]), 6 count (int), 7 sel (2 unsigned char per hit: the loops
- * whose values the real and the imaginary lower medians returned). */
+ * whose values the real and the imaginary lower medians returned), 8 comb keys (RLFC_loops x
+ * W_RLFC bit patterns of |Y|^2: uint64 for SFFT_FP64, uint32 for SFFT_FP32). */
 const void *sfft_plan_debug_buffer(const sfft_plan *plan, int which);
 
 /* ---- the value map and its adjoint (FP64 plans) ----------------------------------------

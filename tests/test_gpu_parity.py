@@ -314,3 +314,19 @@ def test_plan_graph_replays_match_eager(S):
             y = torch.randn(1 << 18, device="cuda")
             (y * 3).sum().item()
             dx.mul_(0.75).add_(torch.roll(dx, 1) * 0.5)
+
+
+@pytest.mark.parametrize("cfg", [SMALL_CONFIGS[1], SMALL_CONFIGS[3], CFG1])
+def test_comb_keys_bitexact(S, cfg):
+    """|Y|^2 of every comb bin (the comb FFT's key epilogue) equals the oracle's samples of
+    its own comb FFT, bit for bit, for both comb loops."""
+    prm, plan, ref, nodes, cnt, _ = _run_both(S, cfg, seed=cfg[0] + 17)
+    n, W, Rl = prm["n"], prm["W_RLFC"], prm["RLFC_loops"]
+    x, _, _ = sparse_signal(n, prm["k"], cfg[0] + 17)
+    _, _, offs = draws(n, prm["loops"], Rl, W, cfg[0] + 17 + 7)
+    keys = plan.debug_array(8, Rl * W, np.uint64).reshape(Rl, W)
+    sigma = n // W
+    for r in range(Rl):
+        y = x[int(offs[r])::sigma][:W]
+        want = O.samples64(O.fft64(y))
+        assert np.array_equal(keys[r], want.view(np.uint64)), r
