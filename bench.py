@@ -33,6 +33,9 @@ CONFIGS = {
     "cfg3": (27, 40000, 0.7, 3, 2, 8, 32768, 2, 30.0),
     # 64 uneven complex64 signals: q uniform on 18..24, k log-uniform on [50, 4000], 40 dB
     "cfg4": None,
+    # larger signals with the cfg2 constants (scaling beyond the headline size)
+    "n28": (28, 16000, 0.65, 2, 2, 7, 4096, 2, None),
+    "n29": (29, 16000, 0.65, 2, 2, 7, 4096, 2, None),
 }
 BATCH = dict(count=64, q_range=(18, 24), k_range=(50, 4000), Bcst=1.0, loops_loc=2,
              loops_thresh=2, loops_est=5, RLFC_cst=256, RLFC_loops=2, snr_db=40.0)
@@ -45,7 +48,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS),
                     help="cfg2 (default): n=2^27 k=16000; cfg1: n=2^20 k=1000; cfg3: n=2^27 "
-                         "k=40000 30 dB; cfg4: a batch of 64 uneven signals")
+                         "k=40000 30 dB; cfg4: a batch of 64 uneven signals; n28/n29: "
+                         "n=2^28/2^29 k=16000 with the cfg2 constants")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--front-end", default="auto", choices=["auto", "gather", "stream"],
                     help="signal access: gathers per tap (auto), or one streaming pass")
