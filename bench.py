@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget for the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=0x5FF70002)
+    ap.add_argument("--design", default="faithful", choices=["faithful", "parallel"],
+                    help="filter design mode (setup, outside the timed region): the "
+                         "reference's rounding, or the device design without recurrences")
     ap.add_argument("--graph", action="store_true",
                     help="replay each transform from a hipGraph instead of enqueuing its "
                          "kernels eagerly (measured equal at n=2^27: launches keep ahead)")
@@ -94,12 +97,13 @@ def main():
     X = S.truth_spectrum(x)
     log(f"[bench] signal + truth spectrum: {time.time() - t:.2f}s")
     t = time.time()
-    ft, ff = S.make_filter(prm["lobefrac"], prm["tolerance"], n, prm["b_f"], w)
+    ft, ff = S.make_filter(prm["lobefrac"], prm["tolerance"], n, prm["b_f"], w,
+                           mode=args.design, freq="device")
     win = S.filter_window(ff, n, B)
     del ff
     if fp32:
         ft = S.to_complex64(ft)
-    log(f"[bench] filter design (FP64, w={w}): {time.time() - t:.2f}s")
+    log(f"[bench] filter design (FP64, {args.design}, w={w}): {time.time() - t:.2f}s")
     a, ai, offs = S.draw_permutations(n, L, prm["RLFC_loops"], prm["W_RLFC"], args.seed + 2)
 
     front = "gather" if args.front_end == "auto" else args.front_end
@@ -213,7 +217,7 @@ def main():
                                f"{'noise ' + str(snr) + 'dB' if snr else 'noiseless'}, "
                                f"{args.precision} path, {front} front end",
                    "n": n, "k": k, "B": B, "W_RLFC": prm["W_RLFC"], "w": w, "loops": L,
-                   "parallelism": "none (1 GPU)"},
+                   "filter_design": args.design, "parallelism": "none (1 GPU)"},
         "median_ms": round(float(np.median(step_ms)), 4),
         "stages_ms": {kk: round(v, 4) for kk, v in stages_ms.items()},
         "intervals_ms": {kk: round(v, 4) for kk, v in iv_ms.items()},
@@ -245,7 +249,8 @@ def main_batch(args):
         xs.append(S.synthesize(n, k, locs, vals, snr_db=snr, noise_seed=args.seed + 2000 + i,
                                fp32=True))    # the batch workload stores complex64
         spectra.append(locs)
-        ft, ff = S.make_filter(p["lobefrac"], p["tolerance"], n, p["b_f"], p["w"])
+        ft, ff = S.make_filter(p["lobefrac"], p["tolerance"], n, p["b_f"], p["w"],
+                               mode=args.design if n >= 4096 else "faithful", freq="device")
         wins.append(S.filter_window(ff, n, p["B"]))
         del ff
         fts.append(S.to_complex64(ft))

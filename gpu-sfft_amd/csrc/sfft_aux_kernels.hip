@@ -85,6 +85,167 @@ __global__ void scale_head_kernel(const double2* __restrict__ g, double2* __rest
   }
 }
 
+// ---- filter design, parallel mode ------------------------------------------------------------
+// The same mathematical filter as the faithful design, computed without sequential recurrences:
+// device Chebyshev samples and Bluestein chirp, a boxcar from two levels of panel sums, a
+// device max, and a phase ramp whose angle is reduced exactly in integers.
+
+// v[i] = (T_m(x_i) * tol, 0) with m = w - 1, x_i = t0 cos(phi), phi = pi i / w, and 0 where
+// x_i < -1 (the faithful design's NaN -> 0 rule).  Evaluated without cancellation: the direct
+// form cosh(m acosh(x)) loses about m^2 eps / 19 relative near x = t0 (4e-5 at w = 2.2M),
+// because x^2 - 1 cancels.  Here x - 1 = (t0 - 1) - 2 t0 sin^2(phi/2) with
+// t0 - 1 = 2 sinh^2(a/2) (t0 = cosh a) comes without cancellation, and then
+//   x > 1:      T_m = cosh(m log1p(d + sqrt(d (2 + d)))),   d = x - 1
+//   0 <= x <= 1: T_m = cos(2 m asin(sqrt((1 - x) / 2)))
+//   -1 <= x < 0: T_m = cos(m (pi - 2 asin(sqrt((1 + x) / 2)))),  1 + x = 2 t0 cos^2(phi/2) - (t0 - 1)
+__global__ void cheb_samples_kernel(double2* __restrict__ v, int w, double t0, double t0m1,
+                                    double tol) {
+  const double m = (double)(w - 1);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < w;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double hq = (double)i / (2.0 * (double)w);   // phi / (2 pi)
+    const double sh = sinpi(hq), ch = cospi(hq);
+    const double d = t0m1 - 2.0 * t0 * (sh * sh);       // x - 1
+    double val;
+    if (d > 0.0) {
+      val = cosh(m * log1p(d + sqrt(d * (2.0 + d))));
+    } else if (d >= -1.0) {
+      val = cos(2.0 * m * asin(sqrt(-d * 0.5)));
+    } else {
+      const double xp1 = 2.0 * t0 * (ch * ch) - t0m1;   // x + 1
+      val = (xp1 >= 0.0) ? cos(m * (M_PI - 2.0 * asin(sqrt(xp1 * 0.5)))) : 0.0;
+    }
+    v[i] = make_double2(val * tol, 0.0);
+  }
+}
+
+// Bluestein chirp c[j] = exp(sgn i pi (j^2 mod 2N) / N), j < N, and its conjugate mirrored into
+// b[0..M): b[j] = b[M-j] = conj(c[j]) for j < N, zero elsewhere (M >= 2N - 1)
+__global__ void chirp_kernel(double2* __restrict__ c, double2* __restrict__ b, int N, long long M,
+                             double sgn) {
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < M;
+       j += (long long)gridDim.x * blockDim.x) {
+    const long long m = (j < N) ? j : ((j > M - N) ? M - j : -1);
+    double2 r = make_double2(0.0, 0.0);
+    if (m >= 0) {
+      const unsigned long long q =
+          ((unsigned long long)m * (unsigned long long)m) % (2ULL * (unsigned long long)N);
+      double s, co;
+      sincospi((double)q / (double)N, &s, &co);
+      if (j < N) c[j] = make_double2(co, sgn * s);
+      r = make_double2(co, -sgn * s);
+    }
+    b[j] = r;
+  }
+}
+
+// ps[p] = sum of in[P p .. P p + P), one thread per panel
+__global__ void panel_sum_kernel(const double2* __restrict__ in, double2* __restrict__ ps,
+                                 long long npan, int P) {
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < npan;
+       p += (long long)gridDim.x * blockDim.x) {
+    double sx = 0.0, sy = 0.0;
+    const double2* src = in + p * P;
+    for (int j = 0; j < P; ++j) {
+      sx += src[j].x;
+      sy += src[j].y;
+    }
+    ps[p] = make_double2(sx, sy);
+  }
+}
+
+constexpr int kPan1 = 64;     // level-1 panel (entries of G)
+constexpr int kPan2 = 4096;   // level-2 panel (64 level-1 panels)
+
+// boxcar s_i = sum of G[i .. i + b_f) (indices mod n) stored at (i + b_f/2) mod n, i.e. the
+// faithful running sum without its n-step rounding drift: each thread owns kPan1 consecutive
+// outputs, starts from a window sum over panel sums (at most 3*63 + b_f/4096 terms) and slides
+// kPan1 - 1 steps.  blkmax[block] = max |s|^2 over the block's outputs.  n >= 4096.
+__global__ __launch_bounds__(256) void boxcar_kernel(const double2* __restrict__ G,
+                                                     const double2* __restrict__ p1,
+                                                     const double2* __restrict__ p2,
+                                                     double2* __restrict__ out, long long n,
+                                                     long long b_f, double* __restrict__ blkmax) {
+  __shared__ double sh[256];
+  const long long mask = n - 1, np1 = n / kPan1, np2 = n / kPan2;
+  const long long c = blockIdx.x * 256LL + threadIdx.x;
+  double mx2 = 0.0;
+  if (c < np1) {
+    const long long i0 = c * kPan1;
+    double sx = 0.0, sy = 0.0;
+    long long pos = i0, len = b_f;
+    while (len > 0) {
+      if ((pos & (kPan2 - 1)) == 0 && len >= kPan2) {
+        const double2 v = p2[(pos / kPan2) % np2];
+        sx += v.x;
+        sy += v.y;
+        pos += kPan2;
+        len -= kPan2;
+      } else if ((pos & (kPan1 - 1)) == 0 && len >= kPan1) {
+        const double2 v = p1[(pos / kPan1) % np1];
+        sx += v.x;
+        sy += v.y;
+        pos += kPan1;
+        len -= kPan1;
+      } else {
+        const double2 v = G[pos & mask];
+        sx += v.x;
+        sy += v.y;
+        ++pos;
+        --len;
+      }
+    }
+    const long long offset = b_f / 2;
+    for (int t = 0; t < kPan1; ++t) {
+      const long long i = i0 + t;
+      out[(i + offset) & mask] = make_double2(sx, sy);
+      const double a2 = sx * sx + sy * sy;
+      mx2 = mx2 > a2 ? mx2 : a2;
+      const double2 ahead = G[(i + b_f) & mask], behind = G[i];
+      sx += ahead.x - behind.x;
+      sy += ahead.y - behind.y;
+    }
+  }
+  sh[threadIdx.x] = mx2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blkmax[blockIdx.x] = sh[0];
+}
+
+// out[0] = max of part[0..np) (one block)
+__global__ __launch_bounds__(256) void max_reduce_kernel(const double* __restrict__ part, int np,
+                                                         double* __restrict__ out) {
+  __shared__ double sh[256];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) m = fmax(m, part[i]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// ff[i] = (ff[i] / sqrt(*mx2)) * exp(-2 pi i (hw i mod n) / n): the normalisation and the phase
+// ramp, with the angle reduced exactly in integers (n a power of two, hw i < 2^63)
+__global__ void ramp_kernel(const double2* in, double2* ff, long long n,
+                            long long hw, const double* __restrict__ mx2) {
+  const double mx = sqrt(*mx2);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long r =
+        ((unsigned long long)hw * (unsigned long long)i) & (unsigned long long)(n - 1);
+    double s, co;
+    sincospi(-2.0 * (double)r / (double)n, &s, &co);
+    const double ax = in[i].x / mx, ay = in[i].y / mx;
+    ff[i] = make_double2(ax * co - ay * s, ax * s + ay * co);
+  }
+}
+
 // ---- signals ---------------------------------------------------------------------------------
 __global__ void scatter_c128_kernel(double2* __restrict__ dst, const int* __restrict__ idx,
                                     const double2* __restrict__ val, int k) {
@@ -238,6 +399,40 @@ hipError_t aux_fold(const double2* ft, double2* g, int w, long long n, hipStream
 }
 hipError_t aux_scale_head(const double2* g, double2* ft, int w, double n, hipStream_t st) {
   hipLaunchKernelGGL(scale_head_kernel, grid_for(w), dim3(256), 0, st, g, ft, w, n);
+  return hipGetLastError();
+}
+hipError_t aux_cheb_samples(double2* v, int w, double tolerance, hipStream_t st) {
+  const double a = acosh(1.0 / tolerance) / (double)(w - 1);
+  const double sh = sinh(0.5 * a);
+  hipLaunchKernelGGL(cheb_samples_kernel, grid_for(w), dim3(256), 0, st, v, w, cosh(a),
+                     2.0 * sh * sh, tolerance);
+  return hipGetLastError();
+}
+hipError_t aux_chirp(double2* c, double2* b, int N, long long M, int inverse, hipStream_t st) {
+  hipLaunchKernelGGL(chirp_kernel, grid_for(M), dim3(256), 0, st, c, b, N, M,
+                     inverse ? 1.0 : -1.0);
+  return hipGetLastError();
+}
+long long aux_boxcar_blocks(long long n) { return (n / kPan1 + 255) / 256; }
+hipError_t aux_boxcar(const double2* G, double2* out, long long n, long long b_f,
+                      double2* panels, double* blkmax, double* mx2, hipStream_t st) {
+  if (n < kPan2 || b_f < 1 || b_f > n) return hipErrorInvalidValue;
+  const long long np1 = n / kPan1, np2 = n / kPan2;
+  double2* p1 = panels;
+  double2* p2 = panels + np1;
+  hipLaunchKernelGGL(panel_sum_kernel, grid_for(np1), dim3(256), 0, st, G, p1, np1, kPan1);
+  hipLaunchKernelGGL(panel_sum_kernel, grid_for(np2), dim3(256), 0, st, (const double2*)p1, p2,
+                     np2, kPan2 / kPan1);
+  const long long nb = aux_boxcar_blocks(n);
+  hipLaunchKernelGGL(boxcar_kernel, dim3((unsigned)nb), dim3(256), 0, st, G,
+                     (const double2*)p1, (const double2*)p2, out, n, b_f, blkmax);
+  hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(256), 0, st, (const double*)blkmax,
+                     (int)nb, mx2);
+  return hipGetLastError();
+}
+hipError_t aux_ramp(const double2* in, double2* ff, long long n, long long hw, const double* mx2,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(ramp_kernel, grid_for(n), dim3(256), 0, st, in, ff, n, hw, mx2);
   return hipGetLastError();
 }
 hipError_t aux_scatter_c128(double2* dst, const int* idx, const double2* val, int k,

@@ -46,8 +46,10 @@ static double cheb(double m, double x) {  // cuCheb, cuExperiment.cu:593-601
 }
 
 // DFT of arbitrary length N on the device: canonical FFT when N is a power of two, else
-// Bluestein with the chirp c[m] = exp(-+ i pi (m^2 mod 2N) / N) built on the host.
-static sfft_status dft_any_device(double2* d_data, int N, int inverse, hipStream_t st) {
+// Bluestein with the chirp c[m] = exp(-+ i pi (m^2 mod 2N) / N), built on the host with libm's
+// sincos (faithful design) or on the device with sincospi (device_chirp).
+static sfft_status dft_any_device(double2* d_data, int N, int inverse, hipStream_t st,
+                                  bool device_chirp = false) {
   sfft_status status = SFFT_OK;
   const int l2 = ilog2_pow2(N);
   void *scr = nullptr, *A = nullptr, *Bv = nullptr, *c = nullptr, *tmp = nullptr;
@@ -64,8 +66,19 @@ static sfft_status dft_any_device(double2* d_data, int N, int inverse, hipStream
     int lm = 0;
     while ((1LL << lm) < 2LL * N - 1) ++lm;
     const long long M = 1LL << lm;
-    std::vector<double2> hc((size_t)N), hb((size_t)M, make_double2(0.0, 0.0));
+    std::vector<double2> hc, hb;
     const double sgn = inverse ? 1.0 : -1.0;
+    HCK(hipMalloc(&c, (size_t)N * 16));
+    HCK(hipMalloc(&A, (size_t)M * 16));
+    HCK(hipMalloc(&Bv, (size_t)M * 16));
+    HCK(hipMalloc(&tmp, (size_t)M * 16));
+    HCK(hipMalloc(&scr, (size_t)M * 16));
+    if (device_chirp) {
+      HCK(aux_chirp((double2*)c, (double2*)tmp, N, M, inverse, st));
+      goto chirp_ready;
+    }
+    hc.resize((size_t)N);
+    hb.assign((size_t)M, make_double2(0.0, 0.0));
     for (long long m = 0; m < N; ++m) {
       const unsigned long long r =
           ((unsigned long long)m * (unsigned long long)m) % (2ULL * (unsigned long long)N);
@@ -80,13 +93,9 @@ static sfft_status dft_any_device(double2* d_data, int N, int inverse, hipStream
       hb[m] = make_double2(hc[m].x, -hc[m].y);
       hb[M - m] = hb[m];
     }
-    HCK(hipMalloc(&c, (size_t)N * 16));
-    HCK(hipMalloc(&A, (size_t)M * 16));
-    HCK(hipMalloc(&Bv, (size_t)M * 16));
-    HCK(hipMalloc(&tmp, (size_t)M * 16));
-    HCK(hipMalloc(&scr, (size_t)M * 16));
     HCK(hipMemcpy(c, hc.data(), (size_t)N * 16, hipMemcpyHostToDevice));
     HCK(hipMemcpy(tmp, hb.data(), (size_t)M * 16, hipMemcpyHostToDevice));
+  chirp_ready:
     HCK(fft_plain(tmp, Bv, scr, lm, 1, 0, 0, st));
     HCK(aux_bluestein_pre(d_data, (const double2*)c, (double2*)tmp, N, M, st));
     HCK(fft_plain(tmp, A, scr, lm, 1, 0, 0, st));
@@ -109,16 +118,67 @@ extern "C" sfft_status sfft_dft_any_device(complex_t* d_data, int N, int inverse
   return dft_any_device((double2*)d_data, N, inverse, (hipStream_t)stream);
 }
 
-// make_multiple_t's filter computation.  d_filter_time: device complex128[w] (output).
-// h_filter_freq: host complex128[n] (output, may be NULL).
-extern "C" sfft_status sfft_make_filter(double lobefrac, double tolerance, int n, int b_f, int w,
+// The parallel design: the same filter with every n-length step on the device and no
+// sequential recurrence (device Chebyshev samples and chirp, panel-sum boxcar, device max,
+// exactly reduced phase ramp).  Not bit-equal to the faithful design: it evaluates the
+// Chebyshev samples without the direct form's cancellation (relative error ~1e-15 instead of
+// up to m^2 eps / 19) and its rounding does not drift with n.
+static sfft_status make_filter_parallel(double tolerance, int n, int ln, int b_f, int w,
                                         complex_t* d_filter_time, complex_t* h_filter_freq,
-                                        void* stream) {
+                                        complex_t* d_filter_freq, hipStream_t st) {
+  sfft_status status = SFFT_OK;
+  void *dv = nullptr, *g = nullptr, *G = nullptr, *scr = nullptr, *pan = nullptr, *mx = nullptr;
+  const long long nb = aux_boxcar_blocks(n);
+  double2* F = nullptr;
+  HCK(hipMalloc(&dv, (size_t)w * 16));
+  HCK(aux_cheb_samples((double2*)dv, w, tolerance, st));
+  status = dft_any_device((double2*)dv, w, 0, st, true);
+  if (status != SFFT_OK) goto done;
+  HCK(aux_shift_real((const double2*)dv, (double2*)d_filter_time, w, st));
+  HCK(hipMalloc(&g, (size_t)n * 16));
+  HCK(hipMalloc(&G, (size_t)n * 16));
+  HCK(hipMalloc(&scr, (size_t)n * 16));
+  HCK(hipMalloc(&pan, (size_t)(n / 64 + n / 4096) * 16));
+  HCK(hipMalloc(&mx, (size_t)(nb + 1) * 8));
+  HCK(aux_fold((const double2*)d_filter_time, (double2*)g, w, n, st));
+  HCK(fft_plain(g, G, scr, ln, 1, 0, 0, st));
+  HCK(aux_boxcar((const double2*)G, (double2*)g, n, b_f, (double2*)pan, (double*)mx + 1,
+                 (double*)mx, st));
+  F = d_filter_freq ? (double2*)d_filter_freq : (double2*)g;
+  HCK(aux_ramp((const double2*)g, F, n, w / 2, (const double*)mx, st));
+  if (h_filter_freq)
+    HCK(hipMemcpyAsync(h_filter_freq, F, (size_t)n * 16, hipMemcpyDeviceToHost, st));
+  HCK(fft_plain(F, G, scr, ln, 1, 1, 0, st));
+  HCK(aux_scale_head((const double2*)G, (double2*)d_filter_time, w, (double)n, st));
+  HCK(hipStreamSynchronize(st));
+done:
+  if (dv) (void)hipFree(dv);
+  if (g) (void)hipFree(g);
+  if (G) (void)hipFree(G);
+  if (scr) (void)hipFree(scr);
+  if (pan) (void)hipFree(pan);
+  if (mx) (void)hipFree(mx);
+  return status;
+}
+
+// make_multiple_t's filter computation.  d_filter_time: device complex128[w] (output).
+// h_filter_freq: host complex128[n], d_filter_freq: device complex128[n] (outputs, may be NULL).
+extern "C" sfft_status sfft_make_filter_ex(double lobefrac, double tolerance, int n, int b_f,
+                                           int w, int mode, complex_t* d_filter_time,
+                                           complex_t* h_filter_freq, complex_t* d_filter_freq,
+                                           void* stream) {
   (void)lobefrac;
   const int ln = ilog2_pow2(n);
   if (ln < 1 || w < 3 || w > n || !d_filter_time || b_f < 1 || b_f > n)
     return SFFT_ERR_INVALID;
+  if (mode != SFFT_DESIGN_FAITHFUL && mode != SFFT_DESIGN_PARALLEL) return SFFT_ERR_INVALID;
+  if (!(tolerance > 0.0 && tolerance < 1.0)) return SFFT_ERR_INVALID;
   hipStream_t st = (hipStream_t)stream;
+  if (mode == SFFT_DESIGN_PARALLEL) {
+    if (n < 4096) return SFFT_ERR_INVALID;
+    return make_filter_parallel(tolerance, n, ln, b_f, w, d_filter_time, h_filter_freq,
+                                d_filter_freq, st);
+  }
   sfft_status status = SFFT_OK;
   void *dv = nullptr, *g = nullptr, *G = nullptr, *scr = nullptr;
   std::vector<double2> hv((size_t)w);
@@ -185,6 +245,8 @@ extern "C" sfft_status sfft_make_filter(double lobefrac, double tolerance, int n
     }
   }
   HCK(hipMemcpy(G, ff, (size_t)n * 16, hipMemcpyHostToDevice));
+  if (d_filter_freq)
+    HCK(hipMemcpyAsync(d_filter_freq, G, (size_t)n * 16, hipMemcpyDeviceToDevice, st));
   HCK(fft_plain(G, g, scr, ln, 1, 1, 0, st));                               // IFFT_n, :424-426
   HCK(aux_scale_head((const double2*)g, (double2*)d_filter_time, w, (double)n, st));  // Kernel2
   HCK(hipStreamSynchronize(st));
@@ -194,6 +256,29 @@ done:
   if (G) (void)hipFree(G);
   if (scr) (void)hipFree(scr);
   return status;
+}
+
+extern "C" sfft_status sfft_make_filter(double lobefrac, double tolerance, int n, int b_f, int w,
+                                        complex_t* d_filter_time, complex_t* h_filter_freq,
+                                        void* stream) {
+  return sfft_make_filter_ex(lobefrac, tolerance, n, b_f, w, SFFT_DESIGN_FAITHFUL,
+                             d_filter_time, h_filter_freq, nullptr, stream);
+}
+
+// win[j] = filter_freq[(j - m/2) mod n] from a device response: two contiguous copies
+extern "C" sfft_status sfft_filter_window_device(const complex_t* d_filter_freq, int n, int B,
+                                                 complex_t* h_win, void* stream) {
+  if (!d_filter_freq || !h_win || ilog2_pow2(n) < 0 || B < 1 || B > n || n % B)
+    return SFFT_ERR_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  const long long m = n / B, h = m / 2;
+  if (h > 0 && hipMemcpyAsync(h_win, d_filter_freq + (n - h), (size_t)h * 16,
+                              hipMemcpyDeviceToHost, st) != hipSuccess)
+    return SFFT_ERR_HIP;
+  if (hipMemcpyAsync(h_win + h, d_filter_freq, (size_t)(m - h) * 16, hipMemcpyDeviceToHost,
+                     st) != hipSuccess)
+    return SFFT_ERR_HIP;
+  return hipStreamSynchronize(st) == hipSuccess ? SFFT_OK : SFFT_ERR_HIP;
 }
 
 extern "C" void sfft_filter_window(const complex_t* filter_freq, int n, int B, complex_t* win) {

@@ -66,7 +66,8 @@ EXPORTED_FUNCTIONS = [
     "sfft_batch_set_graph", "sfft_batch_execute_async", "sfft_batch_execute",
     "sfft_batch_counts_device", "sfft_batch_tune", "sfft_batch_set_shape",
     "sfft_fft_device", "sfft_twiddle", "sfft_fft_plan_passes", "sfft_version",
-    "sfft_device_count", "sfft_filter_taps", "sfft_make_filter", "sfft_filter_window",
+    "sfft_device_count", "sfft_filter_taps", "sfft_make_filter", "sfft_make_filter_ex",
+    "sfft_filter_window", "sfft_filter_window_device",
     "sfft_dft_any_device", "sfft_convert_c128_to_c64", "sfft_convert_c64_to_c128",
     "sfft_signal_spectrum", "sfft_signal_synthesize", "sfft_truth_spectrum", "sfft_metric",
 ]
@@ -154,8 +155,12 @@ def _declare(L):
     L.sfft_filter_taps.restype = _i
     L.sfft_make_filter.argtypes = [_d, _d, _i, _i, _i, _p, _p, _p]
     L.sfft_make_filter.restype = _i
+    L.sfft_make_filter_ex.argtypes = [_d, _d, _i, _i, _i, _i, _p, _p, _p, _p]
+    L.sfft_make_filter_ex.restype = _i
     L.sfft_filter_window.argtypes = [_p, _i, _i, _p]
     L.sfft_filter_window.restype = None
+    L.sfft_filter_window_device.argtypes = [_p, _i, _i, _p, _p]
+    L.sfft_filter_window_device.restype = _i
     L.sfft_dft_any_device.argtypes = [_p, _i, _i, _p]
     L.sfft_dft_any_device.restype = _i
     L.sfft_convert_c128_to_c64.argtypes = [_p, _p, C.c_size_t, _p]

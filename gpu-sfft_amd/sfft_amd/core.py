@@ -421,22 +421,42 @@ def filter_taps(lobefrac, tolerance=1e-8):
     return lib.sfft_filter_taps(lobefrac, tolerance, None)
 
 
-def make_filter(lobefrac, tolerance, n, b_f, w=None):
-    """FP64 filter of make_multiple_t: (filter_time device complex128[w], filter_freq numpy
-    complex128[n])."""
+DESIGN_MODES = {"faithful": 0, "parallel": 1}
+
+
+def make_filter(lobefrac, tolerance, n, b_f, w=None, mode="faithful", freq="host"):
+    """FP64 filter of make_multiple_t: (filter_time device complex128[w], filter_freq).
+
+    mode "faithful" reproduces the reference's sequential boxcar and phase-ramp recurrences bit
+    for bit; "parallel" computes the same filter on the device without recurrences (n >= 4096;
+    different rounding, no drift with n).  freq: "host" returns filter_freq as numpy
+    complex128[n], "device" as a torch complex128[n] on the GPU, None skips it."""
     torch = _torch()
+    if mode not in DESIGN_MODES:
+        raise ValueError(f"mode must be one of {sorted(DESIGN_MODES)}")
+    if freq not in ("host", "device", None):
+        raise ValueError('freq must be "host", "device" or None')
     if w is None:
         w = filter_taps(lobefrac, tolerance)
     ft = torch.empty(w, dtype=torch.complex128, device="cuda")
-    ff = np.empty(n, dtype=np.complex128)
+    ff_h = np.empty(n, dtype=np.complex128) if freq == "host" else None
+    ff_d = torch.empty(n, dtype=torch.complex128, device="cuda") if freq == "device" else None
     torch.cuda.synchronize()
-    _check(lib.sfft_make_filter(lobefrac, tolerance, n, b_f, w, _ptr(ft), _np_ptr(ff),
-                                _stream_handle()), "sfft_make_filter")
-    return ft, ff
+    _check(lib.sfft_make_filter_ex(lobefrac, tolerance, n, b_f, w, DESIGN_MODES[mode], _ptr(ft),
+                                   _np_ptr(ff_h) if ff_h is not None else None,
+                                   _ptr(ff_d) if ff_d is not None else None, _stream_handle()),
+           "sfft_make_filter_ex")
+    return ft, (ff_h if freq == "host" else ff_d)
 
 
 def filter_window(filter_freq, n, B):
+    """win[j] = filter_freq[(j - m/2) mod n], j < m = n/B, from a host or device response."""
     m = n // B
+    if not isinstance(filter_freq, np.ndarray):        # torch tensor on the GPU
+        win = np.empty(m, dtype=np.complex128)
+        _check(lib.sfft_filter_window_device(_ptr(filter_freq), n, B, _np_ptr(win),
+                                             _stream_handle()), "sfft_filter_window_device")
+        return win
     idx = (np.arange(m, dtype=np.int64) - m // 2) % n
     return np.ascontiguousarray(filter_freq[idx])
 

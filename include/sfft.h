@@ -297,8 +297,24 @@ int sfft_filter_taps(double lobefrac, double tolerance, double *t0);
  * boxcar and phase-ramp recurrences).  Synchronous. */
 sfft_status sfft_make_filter(double lobefrac, double tolerance, int n, int b_f, int w,
                              complex_t *d_filter_time, complex_t *h_filter_freq, void *stream);
+/* design modes: FAITHFUL is the computation above (host boxcar and phase-ramp recurrences, bit
+ * for bit the CPU restatement); PARALLEL computes the same filter with every n-length step on
+ * the device (Chebyshev samples, Bluestein chirp, boxcar from panel sums, max, phase ramp with
+ * the angle (w/2) i mod n reduced exactly) -- its rounding differs from FAITHFUL's and does not
+ * drift with n.  PARALLEL needs n >= 4096. */
+typedef enum { SFFT_DESIGN_FAITHFUL = 0, SFFT_DESIGN_PARALLEL = 1 } sfft_design_mode;
+/* sfft_make_filter with a mode and an optional device copy of the response d_filter_freq[n]
+ * (h_filter_freq and d_filter_freq may each be NULL; with both NULL the PARALLEL mode moves no
+ * n-length array to the host).  Replaces make_multiple_t's design (cuExperiment.cu:279-447).
+ * Synchronous. */
+sfft_status sfft_make_filter_ex(double lobefrac, double tolerance, int n, int b_f, int w,
+                                int mode, complex_t *d_filter_time, complex_t *h_filter_freq,
+                                complex_t *d_filter_freq, void *stream);
 /* win[j] = filter_freq[(j - m/2) mod n], j < m = n/B: the entries estimation reads */
 void sfft_filter_window(const complex_t *filter_freq, int n, int B, complex_t *win);
+/* the same window read from a device response (two contiguous copies).  Synchronous. */
+sfft_status sfft_filter_window_device(const complex_t *d_filter_freq, int n, int B,
+                                      complex_t *h_win, void *stream);
 sfft_status sfft_convert_c128_to_c64(const void *d_in, void *d_out, size_t count, void *stream);
 sfft_status sfft_convert_c64_to_c128(const void *d_in, void *d_out, size_t count, void *stream);
 

@@ -177,3 +177,21 @@ def test_permutation_generator_is_glibc_random(S):
 def test_status_strings(S):
     for s in range(7):
         assert S.lib.sfft_status_string(s)
+
+
+def test_make_filter_ex_rejects_bad_arguments(S):
+    """argument checks of sfft_make_filter_ex run before any device call"""
+    import ctypes as C
+    d = C.c_void_p(16)                      # never dereferenced: every call fails validation
+    L = S.lib
+    assert L.sfft_make_filter_ex(1e-3, 1e-8, 1 << 14, 100, 301, 2, d, None, None, None) == 1
+    assert L.sfft_make_filter_ex(1e-3, 1e-8, 2048, 100, 301, 1, d, None, None, None) == 1
+    assert L.sfft_make_filter_ex(1e-3, 1e-8, 3000, 100, 301, 0, d, None, None, None) == 1
+    assert L.sfft_make_filter_ex(1e-3, 1e-8, 1 << 14, 0, 301, 1, d, None, None, None) == 1
+    assert L.sfft_make_filter_ex(1e-3, 1e-8, 1 << 14, 100, 2, 1, d, None, None, None) == 1
+    assert L.sfft_make_filter_ex(1e-3, 2.0, 1 << 14, 100, 301, 1, d, None, None, None) == 1
+    assert L.sfft_make_filter_ex(1e-3, 1e-8, 1 << 14, 100, 301, 1, None, None, None, None) == 1
+    assert L.sfft_filter_window_device(None, 1 << 14, 16, d, None) == 1
+    assert L.sfft_filter_window_device(d, 1 << 14, 3, d, None) == 1
+    with pytest.raises(ValueError):
+        S.make_filter(1e-3, 1e-8, 1 << 14, 100, mode="fast")

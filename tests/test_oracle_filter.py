@@ -75,3 +75,49 @@ def test_filter_window_slice_layout():
     win = O.filter_window_slice(ff, n, B)
     assert win.size == m
     assert win[m // 2] == 0 and win[0] == n - m // 2 and win[m - 1] == m // 2 - 1
+
+
+# ---- the accurate sample form (reference for the parallel design) ---------------------------
+@pytest.mark.parametrize("w", [631, 1849, 123349, 2230607])
+def test_accurate_chebyshev_samples_against_mpmath(w):
+    """The cancellation-free samples against 50-digit evaluations of the definition; the
+    reference's direct form cosh(m acosh(x)) is off by up to ~m^2 eps / 19 near x = t0."""
+    import mpmath as mp
+    from helpers import cheb_samples_accurate
+    mp.mp.dps = 50
+    tol = 1e-8
+    idx = np.unique(np.concatenate([np.arange(40), np.linspace(0, w - 1, 200).astype(np.int64),
+                                    np.arange(w - 12, w)]))
+    m = w - 1
+    a = mp.acosh(1 / mp.mpf(tol)) / m
+    t0 = mp.cosh(a)
+    exact = []
+    for i in idx.tolist():
+        x = t0 * mp.cos(mp.pi * i / w)
+        v = 0 if x < -1 else (mp.cos(m * mp.acos(x)) if abs(x) <= 1 else mp.cosh(m * mp.acosh(x)))
+        exact.append(float(v * mp.mpf(tol)))
+    exact = np.array(exact)
+    peak = np.max(np.abs(exact))
+    acc = cheb_samples_accurate(w, tol)[idx]
+    assert np.max(np.abs(acc - exact)) <= 1e-14 * peak
+    assert np.array_equal(acc == 0, exact == 0)               # the NaN -> 0 region, 6 samples
+    assert np.count_nonzero(exact == 0) == 6
+    t0d = np.cosh(np.arccosh(1 / tol) / (w - 1))
+    x = t0d * np.cos(np.pi * idx / w)
+    direct = np.where(np.abs(x) <= 1, np.cos(m * np.arccos(np.clip(x, -1, 1))),
+                      np.cosh(m * np.arccosh(np.maximum(x, 1.0)))) * tol
+    direct = np.where(x < -1, 0.0, direct)
+    if w > 100000:
+        assert np.max(np.abs(direct - exact)) >= 1e3 * np.max(np.abs(acc - exact))
+
+
+@pytest.mark.parametrize("cfg", SMALL_CONFIGS[:2])
+def test_exact_design_agrees_with_oracle(cfg):
+    """the accurate design differs from the oracle's restatement of the reference's rounding
+    only by the direct sample form's error and the recurrences' drift (small at these n)"""
+    from helpers import exact_filter_design
+    prm = params(cfg)
+    ft, ff = O.make_multiple_t(prm["lobefrac"], prm["tolerance"], prm["n"], prm["b_f"], prm["w"])
+    ft_x, ff_x = exact_filter_design(prm["n"], prm["w"], prm["b_f"], prm["tolerance"])
+    assert np.max(np.abs(ff - ff_x)) <= 1e-11
+    assert np.max(np.abs(ft - ft_x)) <= 1e-11 * np.max(np.abs(ft_x))
