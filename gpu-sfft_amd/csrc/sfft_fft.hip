@@ -511,11 +511,10 @@ __global__ __launch_bounds__(PF_NT) void perm_filter_kernel(
                            blockIdx.x * PF_NT + threadIdx.x, blockIdx.y * LPT);
 }
 
-template <bool TAPS>
-static hipError_t pf_launch(int is_fp32, const void* src, const void* f, void* bins,
-                            const int* ai_dev, int n, int B, int w, int loops, int tiled,
-                            hipStream_t st) {
-  constexpr int LPT = 3;
+template <int LPT, bool TAPS>
+static hipError_t pf_launch_lpt(int is_fp32, const void* src, const void* f, void* bins,
+                                const int* ai_dev, int n, int B, int w, int loops, int tiled,
+                                hipStream_t st) {
   dim3 grid((B + PF_NT - 1) / PF_NT, (loops + LPT - 1) / LPT);
   const uint32_t mask = (uint32_t)n - 1u;
   if (is_fp32)
@@ -527,6 +526,31 @@ static hipError_t pf_launch(int is_fp32, const void* src, const void* f, void* b
                        (const double2*)src, (const double2*)f, (double2*)bins, ai_dev, mask, B,
                        w, loops, tiled);
   return hipGetLastError();
+}
+
+// Loops per thread: as many as possible up to 9, split evenly.  One thread then reads each
+// filter tap once for all its loops and keeps up to 18 independent gathers in flight (the tap
+// loop is unrolled twice); at n = 2^27, k = 16000 (9 loops) this is 7% faster than 3 loops per
+// thread.  The sums do not depend on the split.
+static int pf_loops_per_thread(int loops) {
+  const int groups = (loops + 8) / 9;
+  return (loops + groups - 1) / groups;
+}
+
+template <bool TAPS>
+static hipError_t pf_launch(int is_fp32, const void* src, const void* f, void* bins,
+                            const int* ai_dev, int n, int B, int w, int loops, int tiled,
+                            hipStream_t st) {
+  switch (pf_loops_per_thread(loops)) {
+#define SFFT_PF_LPT(k) \
+  case k:              \
+    return pf_launch_lpt<k, TAPS>(is_fp32, src, f, bins, ai_dev, n, B, w, loops, tiled, st);
+    SFFT_PF_LPT(1) SFFT_PF_LPT(2) SFFT_PF_LPT(3) SFFT_PF_LPT(4) SFFT_PF_LPT(5)
+    SFFT_PF_LPT(6) SFFT_PF_LPT(7) SFFT_PF_LPT(8) SFFT_PF_LPT(9)
+#undef SFFT_PF_LPT
+    default:
+      return hipErrorInvalidValue;
+  }
 }
 
 // Batch: the permute+filter of every signal of a batch in one persistent launch.  Work item
