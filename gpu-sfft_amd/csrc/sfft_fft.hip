@@ -74,17 +74,19 @@ __device__ __forceinline__ void bfly4(T2& a0, T2& a1, T2& a2, T2& a3, int invers
 // ============================================================================================
 // FFT pass
 // ============================================================================================
-constexpr int FFT_NT = 512;
 constexpr int FFT_QMAX = 16;                 // columns per tile
-// complex elements per tile (LDS): 8192 complex64 (66 KB) or 4096 complex128 (66 KB), so
-// that two workgroups share a CU and one loads while the other computes
+// complex elements per tile (LDS): 8192 complex64 (66 KB) or 4096 complex128 (66 KB) with
+// 1024 threads, so that two workgroups (32 waves) fill a CU and one loads while the other
+// computes (1024 threads rather than 512: the n = 2^27 comb and bucket FFT passes run 3-10%
+// faster).
 // The pass that gathers the comb samples from the signal (random lines) uses one 2048-point
-// column per tile (32 KB FP64 / 16 KB FP32), so four workgroups share a CU and more gathers
-// are in flight while other workgroups compute.
+// column per tile (32 KB FP64 / 16 KB FP32) with 512 threads, so four workgroups share a CU
+// and more gathers are in flight while other workgroups compute.
 template <typename Real, int LOAD = 0>
 struct Tile {
   static constexpr int cap = LOAD == 1 ? 2048 : (sizeof(Real) == 8 ? 4096 : 8192);
-  static constexpr int ept = cap / FFT_NT;   // elements per thread
+  static constexpr int nt = LOAD == 1 ? 512 : 1024;   // threads per workgroup
+  static constexpr int ept = cap / nt;                 // elements per thread
 };
 static int tile_cap(int is_fp32, int load) {
   if (load == 1) return Tile<float, 1>::cap;
@@ -132,7 +134,7 @@ __device__ __forceinline__ T2 outer_tw(const PassArgs& A, uint32_t e, int log2Lc
 // stores and the column-interleaved loads spread over the LDS banks.
 #define LPAD(x) ((x) + ((x) >> 4))
 
-template <typename T2, int EPT>
+template <typename T2, int EPT, int NT>
 __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol, int log2ns,
                                            const T2* __restrict__ twR, int inverse) {
   const int tid = threadIdx.x;
@@ -143,7 +145,7 @@ __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol,
   T2 v[EPT];
 #pragma unroll
   for (int kk = 0; kk < EPT / 4; ++kk) {
-    const int bi = tid + kk * FFT_NT;
+    const int bi = tid + kk * NT;
     if (bi < nbf) {
       const int c = bi >> (r - 2);
       const int j = bi & (Rq - 1);
@@ -166,7 +168,7 @@ __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol,
   __syncthreads();
 #pragma unroll
   for (int kk = 0; kk < EPT / 4; ++kk) {
-    const int bi = tid + kk * FFT_NT;
+    const int bi = tid + kk * NT;
     if (bi < nbf) {
       const int c = bi >> (r - 2);
       const int j = bi & (Rq - 1);
@@ -181,7 +183,7 @@ __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol,
   __syncthreads();
 }
 
-template <typename T2, int EPT>
+template <typename T2, int EPT, int NT>
 __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol, int log2ns,
                                            const T2* __restrict__ twR) {
   const int tid = threadIdx.x;
@@ -192,7 +194,7 @@ __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol,
   T2 v[EPT];
 #pragma unroll
   for (int kk = 0; kk < EPT / 2; ++kk) {
-    const int bi = tid + kk * FFT_NT;
+    const int bi = tid + kk * NT;
     if (bi < nbf) {
       const int c = bi >> (r - 1);
       const int j = bi & (Rh - 1);
@@ -206,7 +208,7 @@ __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol,
   __syncthreads();
 #pragma unroll
   for (int kk = 0; kk < EPT / 2; ++kk) {
-    const int bi = tid + kk * FFT_NT;
+    const int bi = tid + kk * NT;
     if (bi < nbf) {
       const int c = bi >> (r - 1);
       const int j = bi & (Rh - 1);
@@ -220,10 +222,11 @@ __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol,
 }
 
 template <typename Real, int LOAD, int STORE>
-__global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
+__global__ __launch_bounds__((Tile<Real, LOAD>::nt)) void fft_pass_kernel(PassArgs A) {
   using T2 = typename CT<Real>::T2;
   using KT = typename CT<Real>::K;
   constexpr int CAP = Tile<Real, LOAD>::cap, EPT = Tile<Real, LOAD>::ept;
+  constexpr int NT = Tile<Real, LOAD>::nt;
   __shared__ T2 lds[CAP + CAP / 16 + FFT_QMAX];
   const int tid = threadIdx.x;
   // The comb-gathering pass runs all comb loops in one grid dimension, loop index fastest,
@@ -265,7 +268,7 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
   const T2* __restrict__ in = (const T2*)A.in + b * A.in_bstride;
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
-    const int e = tid + k * FFT_NT;
+    const int e = tid + k * NT;
     if (e < TE) {
       const int c = e & (ncol - 1);
       const int v = e >> log2ncol;
@@ -290,11 +293,11 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
   int log2ns = 0;
   int rem = r;
   while (rem >= 2) {
-    fft_stage4<T2, EPT>(lds, RP, r, log2ncol, log2ns, twR, A.inverse);
+    fft_stage4<T2, EPT, NT>(lds, RP, r, log2ncol, log2ns, twR, A.inverse);
     log2ns += 2;
     rem -= 2;
   }
-  if (rem == 1) fft_stage2<T2, EPT>(lds, RP, r, log2ncol, log2ns, twR);
+  if (rem == 1) fft_stage2<T2, EPT, NT>(lds, RP, r, log2ncol, log2ns, twR);
 
   // ---- store: out[q*Lcur + t + Lprev*k]
   T2* __restrict__ out = (T2*)A.out + b * A.out_bstride;
@@ -302,7 +305,7 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
   const bool wkeys = (STORE != kStoreCplx) && (b < A.key_batches);
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
-    const int e = tid + k * FFT_NT;
+    const int e = tid + k * NT;
     if (e < TE) {
       int c, kk;
       long long g;
@@ -325,7 +328,7 @@ __global__ __launch_bounds__(FFT_NT) void fft_pass_kernel(PassArgs A) {
 template <typename Real, int LOAD, int STORE>
 static hipError_t launch_pass(const PassArgs& A, int tiles, int batch, hipStream_t st) {
   const dim3 grid = (LOAD == kLoadComb) ? dim3(tiles * batch, 1) : dim3(tiles, batch);
-  hipLaunchKernelGGL((fft_pass_kernel<Real, LOAD, STORE>), grid, dim3(FFT_NT), 0,
+  hipLaunchKernelGGL((fft_pass_kernel<Real, LOAD, STORE>), grid, dim3(Tile<Real, LOAD>::nt), 0,
                      st, A);
   return hipGetLastError();
 }
