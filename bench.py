@@ -112,6 +112,16 @@ def main():
     if args.graph:
         plan.set_graph(True)
 
+    if args.copy_probe_mib > 0:
+        nb = args.copy_probe_mib << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {nb} bytes read, {nb} bytes written")
+        del src, dst
+
     # ---- warm-up, then K timed transforms (Node[I_F] host-visible at the end of each) ----
     for _ in range(args.warmup):
         nodes, cnt = plan.execute(x, a, ai, offs, copy=False)
