@@ -399,11 +399,12 @@ HashArgs hash_args(const sfft_plan* P, int* d_count) {
 // for a single transform; a batch spreads its signals over several streams and runs the
 // permute+filter of all signals as one launch).  The stage-boundary events of the profiling
 // mode are recorded by the front and back groups when `marks` is set.
-// front: vote/bitmap clear, streaming pass (option), comb FFT + cutoff + approved bitmaps
-sfft_status run_front(sfft_plan* P, const void* d_x, const HashArgs& h, hipStream_t st,
-                             bool marks) {
+// front: vote/bitmap clear, streaming pass (option), comb FFT (run_comb), then the comb
+// cutoff and the approved bitmaps (run_cut)
+static sfft_status run_comb(sfft_plan* P, const void* d_x, const HashArgs& h, hipStream_t st,
+                            bool marks) {
   const sfft_params& p = P->p;
-  const int L = p.loops, Rl = p.RLFC_loops, K = p.B_thresh;
+  const int L = p.loops, Rl = p.RLFC_loops;
   const long long W = p.W_RLFC;
   auto mark = [&](int i) -> hipError_t {
     return (marks && P->profile) ? hipEventRecord(P->ev[i], st) : hipSuccess;
@@ -440,11 +441,24 @@ sfft_status run_front(sfft_plan* P, const void* d_x, const HashArgs& h, hipStrea
     c.comb_sigma = (int)(p.n / p.W_RLFC);
   }
   HIPCK(fft_run(c, st));
-  // the comb lists only feed the order-free bitmap build: unordered output
-  HIPCK(select_topk(P->fp32, P->comb_keys, W, (int)W, K, Rl, P->J2, 0, P->selW, st));
-  HIPCK(hash_bitmap_launch(h, st));
-  HIPCK(mark(3));
   return SFFT_OK;
+}
+
+// comb cutoff: the comb lists only feed the order-free bitmap build (unordered output)
+static sfft_status run_cut(sfft_plan* P, const HashArgs& h, hipStream_t st, bool marks) {
+  const sfft_params& p = P->p;
+  const long long W = p.W_RLFC;
+  HIPCK(select_topk(P->fp32, P->comb_keys, W, (int)W, p.B_thresh, p.RLFC_loops, P->J2, 0,
+                    P->selW, st));
+  HIPCK(hash_bitmap_launch(h, st));
+  if (marks && P->profile) HIPCK(hipEventRecord(P->ev[3], st));
+  return SFFT_OK;
+}
+
+sfft_status run_front(sfft_plan* P, const void* d_x, const HashArgs& h, hipStream_t st,
+                      bool marks) {
+  sfft_status v = run_comb(P, d_x, h, st, marks);
+  return v != SFFT_OK ? v : run_cut(P, h, st, marks);
 }
 
 // fold: permutation + filtering of every loop into bins_t
@@ -517,12 +531,13 @@ static sfft_status run_all(sfft_plan* P, const void* d_x, const HashArgs& h, Nod
     if ((v = run_fold(P, d_x, P->st)) != SFFT_OK) return v;
     return run_back(P, h, d_nodes, P->st, true);
   }
+  if ((v = run_comb(P, d_x, h, P->st, false)) != SFFT_OK) return v;
   HIPCK(hipEventRecord(P->ev_fork, P->st));
   HIPCK(hipStreamWaitEvent(P->st2, P->ev_fork, 0));
   if ((v = run_fold(P, d_x, P->st2)) != SFFT_OK) return v;
   if ((v = run_bucket(P, P->st2, false)) != SFFT_OK) return v;
   HIPCK(hipEventRecord(P->ev_join, P->st2));
-  if ((v = run_front(P, d_x, h, P->st, false)) != SFFT_OK) return v;
+  if ((v = run_cut(P, h, P->st, false)) != SFFT_OK) return v;
   HIPCK(hipStreamWaitEvent(P->st, P->ev_join, 0));
   return run_locate(P, h, d_nodes, P->st, false);
 }
