@@ -32,8 +32,8 @@ class Params(C.Structure):
 
 
 class Draws(C.Structure):
-    _fields_ = [("a", C.POINTER(C.c_int)), ("ai", C.POINTER(C.c_int)),
-                ("offsets", C.POINTER(C.c_int))]
+    # host int32 arrays, passed as addresses (sfft_draws: const int *a, *ai, *offsets)
+    _fields_ = [("a", C.c_void_p), ("ai", C.c_void_p), ("offsets", C.c_void_p)]
 
 
 SFFT_OK = 0

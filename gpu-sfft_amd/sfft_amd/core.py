@@ -182,9 +182,7 @@ class Plan:
         self._a = np.ascontiguousarray(a, dtype=np.int32)
         self._ai = np.ascontiguousarray(ai, dtype=np.int32)
         self._off = np.ascontiguousarray(offsets, dtype=np.int32)
-        ip = C.POINTER(C.c_int)
-        return L.Draws(self._a.ctypes.data_as(ip), self._ai.ctypes.data_as(ip),
-                       self._off.ctypes.data_as(ip))
+        return L.Draws(self._a.ctypes.data, self._ai.ctypes.data, self._off.ctypes.data)
 
     def _check_x(self, x):
         torch = _torch()
@@ -358,11 +356,10 @@ class Batch:
                                 "entries")
         ptrs = (C.c_void_p * self.count)(*[x.data_ptr() for x in xs])
         keep, dr = [], (L.Draws * self.count)()
-        ip = C.POINTER(C.c_int)
         for i, (a, ai, off) in enumerate(draws):
             a, ai, off = (np.ascontiguousarray(v, dtype=np.int32) for v in (a, ai, off))
             keep.append((a, ai, off))
-            dr[i] = L.Draws(a.ctypes.data_as(ip), ai.ctypes.data_as(ip), off.ctypes.data_as(ip))
+            dr[i] = L.Draws(a.ctypes.data, ai.ctypes.data, off.ctypes.data)
         return ptrs, dr, keep
 
     def execute_async(self, xs, draws):
