@@ -28,14 +28,14 @@ GATHER_GRANULE = 128
 
 CONFIGS = {
     # name: (q, k, Bcst, loops_loc, loops_thresh, loops_est, RLFC_cst, RLFC_loops, snr_db)
-    "cfg2": (27, 16000, 0.65, 2, 2, 7, 4096, 2, None),
-    "cfg1": (20, 1000, 1.40, 2, 2, 5, 128, 2, None),
-    "cfg3": (27, 40000, 0.7, 3, 2, 8, 32768, 2, 30.0),
+    "n27_k16000": (27, 16000, 0.65, 2, 2, 7, 4096, 2, None),   # the reference README's run
+    "n20_k1000": (20, 1000, 1.40, 2, 2, 5, 128, 2, None),
+    "n27_k40000_30db": (27, 40000, 0.7, 3, 2, 8, 32768, 2, 30.0),
     # 64 uneven complex64 signals: q uniform on 18..24, k log-uniform on [50, 4000], 40 dB
-    "cfg4": None,
-    # larger signals with the cfg2 constants (scaling beyond the headline size)
-    "n28": (28, 16000, 0.65, 2, 2, 7, 4096, 2, None),
-    "n29": (29, 16000, 0.65, 2, 2, 7, 4096, 2, None),
+    "batch64": None,
+    # larger signals with the n27_k16000 constants (scaling beyond the headline size)
+    "n28_k16000": (28, 16000, 0.65, 2, 2, 7, 4096, 2, None),
+    "n29_k16000": (29, 16000, 0.65, 2, 2, 7, 4096, 2, None),
 }
 BATCH = dict(count=64, q_range=(18, 24), k_range=(50, 4000), Bcst=1.0, loops_loc=2,
              loops_thresh=2, loops_est=5, RLFC_cst=256, RLFC_loops=2, snr_db=40.0)
@@ -46,10 +46,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS),
-                    help="cfg2 (default): n=2^27 k=16000; cfg1: n=2^20 k=1000; cfg3: n=2^27 "
-                         "k=40000 30 dB; cfg4: a batch of 64 uneven signals; n28/n29: "
-                         "n=2^28/2^29 k=16000 with the cfg2 constants")
+    ap.add_argument("--config", default="n27_k16000", choices=sorted(CONFIGS),
+                    help="workload: n27_k16000 (default; n=2^27, k=16000), n20_k1000, "
+                         "n27_k40000_30db (30 dB noise), batch64 (64 uneven signals), "
+                         "n28_k16000 / n29_k16000 (the n27_k16000 constants at larger n)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--front-end", default="auto", choices=["auto", "gather", "stream"],
                     help="signal access: gathers per tap (auto), or one streaming pass")
@@ -80,7 +80,7 @@ def main():
     import sfft_amd as S
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
-    if args.config == "cfg4":
+    if args.config == "batch64":
         return main_batch(args)
     q, k, Bcst, ll, lt, le, rc, rl, snr = CONFIGS[args.config]
     prm = S.derive_params(q, k, Bcst, ll, lt, le, rc, rl)
@@ -218,7 +218,7 @@ def main():
         "metric": METRIC, "value": round(ms_per_step, 4), "unit": "ms", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": round(ms_per_step / PUBLISHED_MS, 6) if args.config == "cfg2" and
+        "vs_baseline": round(ms_per_step / PUBLISHED_MS, 6) if args.config == "n27_k16000" and
         not fp32 else None,
         "dtype": "f32" if fp32 else "f64", "data": "synthetic", "front_end": front,
         "launch": "hipGraph" if args.graph else "eager",
@@ -268,7 +268,7 @@ def main_batch(args):
         draws.append(S.draw_permutations(n, p["loops"], p["RLFC_loops"], p["W_RLFC"],
                                          args.seed + 3000 + i))
     ns = [p["n"] for p in prms]
-    log(f"[bench] cfg4: {count} signals, n 2^{min(ns).bit_length() - 1}..2^"
+    log(f"[bench] batch64: {count} signals, n 2^{min(ns).bit_length() - 1}..2^"
         f"{max(ns).bit_length() - 1}, {sum(ns) / 2**20:.0f} Mi samples, k "
         f"{min(p['k'] for p in prms)}..{max(p['k'] for p in prms)}; inputs + filters "
         f"{time.time() - t:.1f}s")
@@ -310,7 +310,7 @@ def main_batch(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "cfg4: 64 complex64 signals, n=2^18..2^24, k log-uniform "
+        "config": {"workload": "batch64: 64 complex64 signals, n=2^18..2^24, k log-uniform "
                                "50..4000, 40 dB noise, Bcst=1.0 loops=2+5 RLFC=256x2",
                    "signals": count, "samples": sum(ns), "parallelism": "none (1 GPU)"},
         "eager_ms": round(eager_ms, 4), "tuned_shape": choice,

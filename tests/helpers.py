@@ -11,8 +11,8 @@ SMALL_CONFIGS = [
     (16, 50, 1.0, 3, 2, 4, 16, 2),
     (17, 100, 0.8, 2, 1, 3, 32, 1),
 ]
-CFG1 = (20, 1000, 1.40, 2, 2, 5, 128, 2)
-CFG2 = (27, 16000, 0.65, 2, 2, 7, 4096, 2)    # BASELINE configs[1] (README sample)
+N20_K1000 = (20, 1000, 1.40, 2, 2, 5, 128, 2)
+N27_K16000 = (27, 16000, 0.65, 2, 2, 7, 4096, 2)   # the reference README's sample run
 
 
 def params(cfg):

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import CFG1, CFG2, SMALL_CONFIGS
+from helpers import N20_K1000, N27_K16000, SMALL_CONFIGS
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "sfft.h")
@@ -63,7 +63,7 @@ def test_timesmod(S):
 
 
 def test_params_match_oracle(S):
-    for cfg in SMALL_CONFIGS + [CFG1, CFG2, (18, 2048, 1.0, 2, 2, 5, 256, 2)]:
+    for cfg in SMALL_CONFIGS + [N20_K1000, N27_K16000, (18, 2048, 1.0, 2, 2, 5, 256, 2)]:
         a = S.derive_params(*cfg)
         b = O.derive_params(*cfg)
         for key in ("n", "B", "B_thresh", "W_RLFC", "w", "b_f", "BB", "lobefrac", "t0", "loops"):
@@ -136,7 +136,7 @@ def test_twiddles_match_oracle_bitwise(S):
 
 
 def test_filter_taps_match(S):
-    for cfg in SMALL_CONFIGS + [CFG1, CFG2]:
+    for cfg in SMALL_CONFIGS + [N20_K1000, N27_K16000]:
         p = O.derive_params(*cfg)
         assert S.filter_taps(p["lobefrac"], p["tolerance"]) == p["w"]
 

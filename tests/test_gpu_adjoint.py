@@ -5,13 +5,13 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import CFG1, SMALL_CONFIGS, bits, draws, oracle_filter, params, sparse_signal
+from helpers import N20_K1000, SMALL_CONFIGS, bits, draws, oracle_filter, params, sparse_signal
 
 pytestmark = pytest.mark.gpu
 
 # n=2^22, k=1000, Bcst=0.99, loops 2+5, thresh 2, RLFC 256 x 2 (the differentiable-path
 # workload)
-CFG5 = (22, 1000, 0.99, 2, 2, 5, 256, 2)
+N22_K1000 = (22, 1000, 0.99, 2, 2, 5, 256, 2)
 
 
 @pytest.fixture(scope="module")
@@ -41,9 +41,9 @@ def _setup(S, cfg, seed, fold_mode=O.FOLD_REFERENCE):
 
 
 @pytest.mark.parametrize("fold_mode", [O.FOLD_REFERENCE, O.FOLD_ALL_TAPS])
-@pytest.mark.parametrize("cfg", SMALL_CONFIGS[:3] + [CFG1])
+@pytest.mark.parametrize("cfg", SMALL_CONFIGS[:3] + [N20_K1000])
 def test_value_map_and_adjoint_bitexact(S, cfg, fold_mode):
-    if cfg == CFG1 and fold_mode == O.FOLD_ALL_TAPS:
+    if cfg == N20_K1000 and fold_mode == O.FOLD_ALL_TAPS:
         pytest.skip("covered by the small configurations")
     prm, x, ft, win, a, ai, offs, plan, _ = _setup(S, cfg, 61, fold_mode)
     dx = _dev(x)
@@ -100,14 +100,14 @@ def _loss(vals, w):
     return float(np.sum(w * np.abs(vals) ** 2))
 
 
-def test_autograd_cfg5_finite_differences_and_dense_gradient(S):
+def test_autograd_n22_k1000_finite_differences_and_dense_gradient(S):
     """n=2^22, k=1000 (complex64 signal, differentiated in FP64): dLoss/dx of
     Loss = sum_j w_j |Xhat_j|^2, w_j ~ U(0,1), by the adjoint kernels, against FP64 central
     finite differences (step 1e-6) at 256 random signal indices with the located indices and
     median choices held fixed, and against the exact gradient of the dense DFT coefficients
     at the located indices."""
     import torch
-    prm = params(CFG5)
+    prm = params(N22_K1000)
     n, k, B = prm["n"], prm["k"], prm["B"]
     x, locs, _ = sparse_signal(n, k, 2022)
     x = x.astype(np.complex64).astype(np.complex128)
@@ -155,7 +155,7 @@ def test_autograd_cfg5_finite_differences_and_dense_gradient(S):
     scale = np.max(np.abs(g))
     L0 = float(loss.item())
     rel = {h: float(np.max(np.abs(f - g)) / scale) for h, f in fd.items()}
-    print(f"cfg5 |Loss| = {L0:.3f}, max |dLoss/dx| = {scale:.3e}; central differences: "
+    print(f"n=2^22 k=1000 |Loss| = {L0:.3f}, max |dLoss/dx| = {scale:.3e}; central differences: "
           f"max |fd - grad| / max |grad| = {rel[1e-6]:.3e} (step 1e-6), "
           f"{rel[1e-2]:.3e} (step 1e-2)", flush=True)
     # rounding bound of a 1e-6 difference: a few ulps of the loss over 2h
@@ -180,6 +180,6 @@ def test_autograd_cfg5_finite_differences_and_dense_gradient(S):
         spec = np.fft.fft(xt2.grad.cpu().numpy())[keys_np[mask]]
         dense = 2 * w[mask] * Xh[keys_np[mask]]
         rel_dense[name] = float(np.linalg.norm(spec - dense) / np.linalg.norm(dense))
-    print("cfg5 dense-DFT gradient at the located indices, relative error: "
+    print("n=2^22 k=1000 dense-DFT gradient at the located indices, relative error: "
           + ", ".join(f"{k_}: {v_:.3e}" for k_, v_ in rel_dense.items()), flush=True)
     assert rel_dense["true locations"] <= 0.1

@@ -15,7 +15,7 @@ import time
 import numpy as np
 import pytest
 
-from helpers import CFG1, CFG2, SMALL_CONFIGS, exact_filter_design, params
+from helpers import N20_K1000, N27_K16000, SMALL_CONFIGS, exact_filter_design, params
 
 pytestmark = pytest.mark.gpu
 
@@ -48,7 +48,7 @@ def _errors(S, n, w, b_f):
     return e
 
 
-@pytest.mark.parametrize("cfg", SMALL_CONFIGS + [CFG1])
+@pytest.mark.parametrize("cfg", SMALL_CONFIGS + [N20_K1000])
 def test_parallel_design_matches_exact(S, cfg):
     prm = params(cfg)
     e = _errors(S, prm["n"], prm["w"], prm["b_f"])
@@ -56,7 +56,7 @@ def test_parallel_design_matches_exact(S, cfg):
     assert e["ff_p"] <= 1e-13 and e["ft_p"] <= 1e-13
     assert e["peak"] == pytest.approx(1.0, abs=1e-14)
     # the faithful design carries the direct sample form's error (3.5e-8 relative in the
-    # samples at w = 123349, CFG1)
+    # samples at w = 123349, N20_K1000)
     assert e["ff_f"] <= 1e-7
 
 
@@ -100,7 +100,7 @@ def _transform(S, prm, ft, win, x, d, locs, X):
     return nodes, cnt, S.metric(nodes, prm["k"], locs, X)
 
 
-@pytest.mark.parametrize("cfg", [CFG1, CFG2])
+@pytest.mark.parametrize("cfg", [N20_K1000, N27_K16000])
 def test_parallel_filter_transform_agrees(S, cfg):
     import torch
     prm = params(cfg)
@@ -131,7 +131,7 @@ def test_parallel_filter_transform_agrees(S, cfg):
     print(f"  hits {cnt_p} / {cnt_f}; MSE*k parallel {m_p['mse_k']:.6e} faithful "
           f"{m_f['mse_k']:.6e}; max value difference at true frequencies {diff:.2e}", flush=True)
     # every true frequency located with either filter; the result's top-k misses (a property
-    # of the configuration: one at CFG1 with this signal) are the same
+    # of the configuration: one at N20_K1000 with this signal) are the same
     assert m_p["missed_est"] == 0 and m_f["missed_est"] == 0
     assert m_p["missed_res"] == m_f["missed_res"]
     assert dff <= 1e-4

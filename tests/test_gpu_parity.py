@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import CFG1, SMALL_CONFIGS, bits, draws, oracle_filter, params, sparse_signal
+from helpers import N20_K1000, SMALL_CONFIGS, bits, draws, oracle_filter, params, sparse_signal
 
 pytestmark = pytest.mark.gpu
 
@@ -104,12 +104,12 @@ def test_transform_all_taps_bitexact(S, cfg):
     _assert_transform_equal(prm, plan, ref, nodes, cnt)
 
 
-@pytest.mark.parametrize("cfg", SMALL_CONFIGS + [CFG1])
+@pytest.mark.parametrize("cfg", SMALL_CONFIGS + [N20_K1000])
 @pytest.mark.parametrize("fold_mode", [O.FOLD_REFERENCE, O.FOLD_ALL_TAPS])
 def test_transform_stream_front_end_bitexact(S, cfg, fold_mode):
     """Streaming front end (one pass over x fills the tap rows and the comb samples, then the
     fold reads the rows): same arithmetic as the gather path, so the same bits."""
-    if fold_mode == O.FOLD_ALL_TAPS and cfg == CFG1:
+    if fold_mode == O.FOLD_ALL_TAPS and cfg == N20_K1000:
         pytest.skip("covered by the small configurations")
     prm, plan, ref, nodes, cnt, _ = _run_both(S, cfg, seed=cfg[0] + 3, fold_mode=fold_mode,
                                               front_end="stream")
@@ -121,9 +121,9 @@ def test_transform_noisy_bitexact(S):
     _assert_transform_equal(prm, plan, ref, nodes, cnt)
 
 
-def test_transform_cfg1_bitexact(S):
+def test_transform_n20_k1000_bitexact(S):
     """n=2^20, k=1000 (the reference's CPU-runnable configuration)."""
-    prm, plan, ref, nodes, cnt, locs = _run_both(S, CFG1, seed=2024)
+    prm, plan, ref, nodes, cnt, locs = _run_both(S, N20_K1000, seed=2024)
     _assert_transform_equal(prm, plan, ref, nodes, cnt)
     assert set(locs.tolist()) <= set(nodes["key"].tolist())
 
@@ -262,7 +262,7 @@ def test_reference_outer_loop_entry_point(S):
 
 
 # ---- the FP32 path: complex64 storage, FP32 FFTs and Kahan fold, FP64 estimation ---------
-@pytest.mark.parametrize("cfg", [SMALL_CONFIGS[1], SMALL_CONFIGS[3], CFG1])
+@pytest.mark.parametrize("cfg", [SMALL_CONFIGS[1], SMALL_CONFIGS[3], N20_K1000])
 @pytest.mark.parametrize("front_end", ["gather", "stream"])
 def test_transform_fp32_bitexact(S, cfg, front_end):
     prm = params(cfg)
@@ -316,7 +316,7 @@ def test_plan_graph_replays_match_eager(S):
             dx.mul_(0.75).add_(torch.roll(dx, 1) * 0.5)
 
 
-@pytest.mark.parametrize("cfg", [SMALL_CONFIGS[1], SMALL_CONFIGS[3], CFG1])
+@pytest.mark.parametrize("cfg", [SMALL_CONFIGS[1], SMALL_CONFIGS[3], N20_K1000])
 def test_comb_keys_bitexact(S, cfg):
     """|Y|^2 of every comb bin (the comb FFT's key epilogue) equals the oracle's samples of
     its own comb FFT, bit for bit, for both comb loops."""

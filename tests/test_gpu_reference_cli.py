@@ -56,7 +56,7 @@ def test_reference_cli_readme_sample():
     assert _result(_run(["27", "16000", "0.65", "2", "2", "7", "4096", "2"]))[2] == msek
 
 
-def test_reference_cli_cfg1():
+def test_reference_cli_n20_k1000():
     out = _run(["20", "1000", "1.40", "2", "2", "5", "128", "2"])
     missed_e, missed_r, _, mse = _result(out)
     assert (missed_e, missed_r) == (0, 0)

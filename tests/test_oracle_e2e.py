@@ -3,11 +3,11 @@ the harness metric (cuExperiment.cu:181-221) on hand-built cases."""
 import numpy as np
 
 import oracle as O
-from helpers import CFG1, draws, oracle_filter, params, sparse_signal
+from helpers import N20_K1000, draws, oracle_filter, params, sparse_signal
 
 
-def test_cfg1_recovers_every_location():
-    prm = params(CFG1)
+def test_n20_k1000_recovers_every_location():
+    prm = params(N20_K1000)
     n, k = prm["n"], prm["k"]
     x, locs, vals = sparse_signal(n, k, 2024)
     ft, ff = oracle_filter(prm)

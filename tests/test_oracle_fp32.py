@@ -4,7 +4,7 @@ few FP32 ulps of the signal scale."""
 import numpy as np
 
 import oracle as O
-from helpers import CFG1, SMALL_CONFIGS, draws, oracle_filter, params, sparse_signal
+from helpers import N20_K1000, SMALL_CONFIGS, draws, oracle_filter, params, sparse_signal
 
 
 def _as_dict(nodes):
@@ -12,7 +12,7 @@ def _as_dict(nodes):
 
 
 def test_fp32_path_matches_fp64_path():
-    for cfg, seed in ((CFG1, 2024), (SMALL_CONFIGS[3], 5)):
+    for cfg, seed in ((N20_K1000, 2024), (SMALL_CONFIGS[3], 5)):
         prm = params(cfg)
         n, k = prm["n"], prm["k"]
         x, locs, vals = sparse_signal(n, k, seed)

@@ -7,12 +7,12 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import CFG1, CFG2, SMALL_CONFIGS
+from helpers import N20_K1000, N27_K16000, SMALL_CONFIGS
 
 
 def test_readme_sample_parameters():
     # ./exec 27 16000 0.65 2 2 7 4096 2  ->  README.md:84-90
-    p = O.derive_params(*CFG2)
+    p = O.derive_params(*N27_K16000)
     assert p["n"] == 134217728
     assert p["B_thresh"] == 32000
     assert p["W_RLFC"] == 4194304
@@ -23,8 +23,8 @@ def test_readme_sample_parameters():
     assert p["assert_ok"] == 1
 
 
-def test_cfg1_closed_forms():
-    p = O.derive_params(*CFG1)
+def test_n20_k1000_closed_forms():
+    p = O.derive_params(*N20_K1000)
     n, k = 1 << 20, 1000
     BB = float(int(1.40 * math.sqrt(n * k / 20)))
     assert p["BB"] == BB == 10137.0
@@ -33,7 +33,7 @@ def test_cfg1_closed_forms():
 
 
 def test_small_configs_satisfy_reference_asserts():
-    for cfg in SMALL_CONFIGS + [CFG1, CFG2]:
+    for cfg in SMALL_CONFIGS + [N20_K1000, N27_K16000]:
         p = O.derive_params(*cfg)
         assert p["assert_ok"] == 1, cfg
         assert p["B_thresh"] < p["B"] and p["B_thresh"] < p["W_RLFC"]
