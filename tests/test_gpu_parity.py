@@ -128,14 +128,17 @@ def test_transform_n20_k1000_bitexact(S):
     assert set(locs.tolist()) <= set(nodes["key"].tolist())
 
 
-@pytest.mark.parametrize("kind", ["constant", "one_tone", "integer"])
+@pytest.mark.parametrize("kind", ["zero", "constant", "one_tone", "integer"])
 def test_transform_degenerate_signals_ties(S, kind):
-    """Signals whose comb and bucket spectra are mostly exact zeros: the cutoffs must pick the
-    lowest indices among the tied keys exactly like the oracle."""
+    """Signals whose comb and bucket spectra are mostly (or, for the zero signal, entirely)
+    exact zeros: the cutoffs must pick the lowest indices among the tied keys exactly like the
+    oracle."""
     prm = params(SMALL_CONFIGS[1])
     n, B = prm["n"], prm["B"]
     rng = np.random.default_rng(17)
-    if kind == "constant":
+    if kind == "zero":
+        x = np.zeros(n, dtype=np.complex128)
+    elif kind == "constant":
         x = np.ones(n, dtype=np.complex128)
     elif kind == "one_tone":
         x = np.exp(2j * np.pi * 123 * np.arange(n) / n)
