@@ -41,7 +41,8 @@ struct KeyBits<unsigned int> {
   static constexpr int KB = 31;
 };
 
-__device__ __forceinline__ int ilog2_dev(int N) { return 31 - __clz(N); }
+// bits of the index field of a composite key: enough for N - 1 (any N, not only powers of 2)
+__device__ __forceinline__ int index_bits(int N) { return N > 1 ? 32 - __clz(N - 1) : 0; }
 
 
 // elements per block: about 256 blocks per segment for large N, at least 2048
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
   __shared__ int s_digit, s_above, s_exit;
   __shared__ int s_out;
   const int tid = threadIdx.x, seg = blockIdx.x;
-  const int IB = ilog2_dev(N);
+  const int IB = index_bits(N);
   const int TB = KeyBits<K>::KB + IB;
   const int nc = st[seg].cand_count;
   const K* ck = ckey + (long long)seg * N;
@@ -471,7 +472,7 @@ __global__ __launch_bounds__(SEL_NT) void sel_count_kernel(const K* __restrict__
                                                            int* __restrict__ cnt) {
   __shared__ int sh[16];
   const int seg = blockIdx.y, tid = threadIdx.x;
-  const int IB = ilog2_dev(N);
+  const int IB = index_bits(N);
   const u128 T = ((u128)st[seg].thr_hi << 64) | (u128)st[seg].thr_lo;
   const K* kp = keys + (long long)seg * seg_stride;
   const int begin = blockIdx.x * chunk, end = min(N, begin + chunk);
@@ -500,7 +501,7 @@ __global__ __launch_bounds__(SEL_NT) void sel_write_kernel(const K* __restrict__
   __shared__ int sh[16];
   __shared__ int s_base, s_tot;
   const int seg = blockIdx.y, tid = threadIdx.x;
-  const int IB = ilog2_dev(N);
+  const int IB = index_bits(N);
   const u128 T = ((u128)st[seg].thr_hi << 64) | (u128)st[seg].thr_lo;
   const K* kp = keys + (long long)seg * seg_stride;
   if (tid == 0) {

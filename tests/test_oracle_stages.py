@@ -96,7 +96,8 @@ def _cutoff_brute(s, K):
     return np.sort(order[:K]).astype(np.int32)
 
 
-@pytest.mark.parametrize("N,K", [(64, 10), (1024, 200), (8192, 2000), (1 << 16, 1000)])
+@pytest.mark.parametrize("N,K", [(64, 10), (1024, 200), (8192, 2000), (1 << 16, 1000), (1000, 100),
+                                 (5000, 1234), (70001, 30000)])
 def test_cutoff_brute_force_with_ties(N, K):
     rng = np.random.default_rng(N + K)
     for s in (rng.random(N),                                   # distinct
