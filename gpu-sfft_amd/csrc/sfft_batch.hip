@@ -294,7 +294,8 @@ extern "C" sfft_status sfft_batch_tune(sfft_batch* b, const void* const* d_x,
   sfft_status v = prepare(b, d_x, draws);
   if (v != SFFT_OK) return v;
   // 1. permute+filter: loops per thread x block size x persistent grid
-  const int lpts[] = {1, 2, 3, 4}, nts[] = {128, 256}, grids[] = {1024, 2048, 4096};
+  const int lpts[] = {1, 2, 3, 4, 5, 6, 7, 8, 9}, nts[] = {128, 256},
+            grids[] = {1024, 2048, 4096};
   float best = 1e30f;
   int bl = b->lpt, bn = b->nt, bg = b->grid;
   for (int lpt : lpts)
@@ -385,11 +386,11 @@ extern "C" sfft_status sfft_batch_tune(sfft_batch* b, const void* const* d_x,
   return SFFT_OK;
 }
 
-// explicit shape (for tests that compare shapes): lpt in 1..4, nt 128/256, grid >= 1,
+// explicit shape (for tests that compare shapes): lpt in 1..9, nt 128/256, grid >= 1,
 // FFT columns per tile (0 = default) for the bucket and comb FFTs
 extern "C" sfft_status sfft_batch_set_shape(sfft_batch* b, int lpt, int nt, int grid,
                                             int qmax_bucket, int qmax_comb) {
-  if (!b || lpt < 1 || lpt > 4 || (nt != 128 && nt != 256) || grid < 1 || qmax_bucket < 0 ||
+  if (!b || lpt < 1 || lpt > 9 || (nt != 128 && nt != 256) || grid < 1 || qmax_bucket < 0 ||
       qmax_bucket > 16 || qmax_comb < 0 || qmax_comb > 16)
     return SFFT_ERR_INVALID;
   drop_graph(b);

@@ -606,6 +606,11 @@ static hipError_t pfb_lpt(const PfJob* jobs, int njobs, int total, int lpt, int 
     case 2: return pfb_nt<Real, 2>(jobs, njobs, total, nt, grid, st);
     case 3: return pfb_nt<Real, 3>(jobs, njobs, total, nt, grid, st);
     case 4: return pfb_nt<Real, 4>(jobs, njobs, total, nt, grid, st);
+    case 5: return pfb_nt<Real, 5>(jobs, njobs, total, nt, grid, st);
+    case 6: return pfb_nt<Real, 6>(jobs, njobs, total, nt, grid, st);
+    case 7: return pfb_nt<Real, 7>(jobs, njobs, total, nt, grid, st);
+    case 8: return pfb_nt<Real, 8>(jobs, njobs, total, nt, grid, st);
+    case 9: return pfb_nt<Real, 9>(jobs, njobs, total, nt, grid, st);
   }
   return hipErrorInvalidValue;
 }
