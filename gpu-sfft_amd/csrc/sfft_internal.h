@@ -1,5 +1,6 @@
-// sfft_internal.h -- shared declarations between the HIP kernels (sfft_kernels.hip) and the
-// host orchestration (sfft_plan.hip).  Not part of the public ABI.
+// sfft_internal.h -- shared declarations between the HIP kernels (sfft_fft.hip,
+// sfft_select.hip, sfft_hash.hip, sfft_adjoint.hip) and the host orchestration
+// (sfft_plan.hip, sfft_batch.hip, sfft_compat.hip).  Not part of the public ABI.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -111,7 +112,8 @@ struct SelectWork {
   int max_segs = 0;
   int max_chunks = 0;
 };
-// N must be a power of two; the workspace must start zeroed
+// any N >= 2 (the composite key's index field takes ceil(log2 N) bits); the workspace must
+// start zeroed
 size_t select_work_bytes(int segs, int N);
 void select_work_bind(SelectWork* w, void* base, int segs, int N);
 // out[s*K + j]: the K largest keys of segment s (ties by lower index); ascending indices when
