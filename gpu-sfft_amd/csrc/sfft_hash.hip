@@ -130,7 +130,8 @@ __host__ __device__ __forceinline__ int hash_lpj(long long m) {
 // [low + d_begin, low + d_begin + d_len) of bucket J's window [J m - m/2, J m + m/2) (mod n);
 // the whole window is empty when (low mod W) exceeds the largest approved residue (the
 // reference's findUpperBound returns the last entry and the walk stops without wrapping).
-// Calls fn(loc) with loc = p * a mod n.
+// Calls fn(loc, rel) with loc = p * a mod n and rel = p - (low + d_begin), the position in
+// the slice.
 template <typename F>
 __device__ __forceinline__ void hash_walk(const HashArgs& h, int l, int i, long long d_begin,
                                           long long d_len, F&& fn) {
@@ -157,7 +158,7 @@ __device__ __forceinline__ void hash_walk(const HashArgs& h, int l, int i, long 
       const int bit = __ffs(word) - 1;
       word &= word - 1u;
       const uint64_t pp = p + (uint64_t)bit;
-      fn((uint32_t)((pp * a) & nmask));
+      fn((uint32_t)((pp * a) & nmask), (int)(d - d_begin) + bit);
     }
     d += take;
   }
@@ -167,6 +168,14 @@ constexpr int HASH_NT = 256;
 
 int hash_blocks(int loops_loc, int K, long long m) {
   return (int)(((long long)loops_loc * K * hash_lpj(m) + HASH_NT - 1) / HASH_NT);
+}
+
+// 32-bit words of one slice's hit mask (a slice covers m / lpj window positions)
+__host__ __device__ __forceinline__ int hash_slice_words(long long m) {
+  return (int)((m / hash_lpj(m) + 31) / 32);
+}
+size_t hash_mask_bytes(int loops_loc, int K, long long m) {
+  return (size_t)hash_blocks(loops_loc, K, m) * HASH_NT * hash_slice_words(m) * 4;
 }
 
 // A location is voted at most once per loop (the J windows of a loop are disjoint), so its
@@ -214,7 +223,7 @@ __global__ __launch_bounds__(HASH_NT) void hash_vote_kernel(HashArgs h, int l_fi
   if (!s.valid) return;
   const int lgepw = 5 - __ffs(h.bpe) + 1;  // log2(32 / bpe)
   const uint32_t epw_mask = (1u << lgepw) - 1u;
-  hash_walk(h, s.l, s.i, s.d0, s.dl, [&](uint32_t loc) {
+  hash_walk(h, s.l, s.i, s.d0, s.dl, [&](uint32_t loc, int) {
     atomicOr(&h.vote[loc >> lgepw], (1u << s.l) << ((loc & epw_mask) * (uint32_t)h.bpe));
   });
 }
@@ -239,20 +248,39 @@ __device__ __forceinline__ bool hash_is_hit(const HashArgs& h, int l, uint32_t l
   return __popc(mask & ((1u << l) - 1u)) + 1 == h.loops_thresh;
 }
 
-__device__ __forceinline__ int hash_count_slice(const HashArgs& h, const HashSlot& s) {
+// hit count of a slot's slice, and its hit mask (bit rel of word rel/32 set for a hit at
+// slice position rel) into mk[0 .. words)
+__device__ __forceinline__ int hash_count_slice(const HashArgs& h, const HashSlot& s,
+                                                unsigned* __restrict__ mk, int words) {
   const int lgepw = 5 - __ffs(h.bpe) + 1;
   const uint32_t epw_mask = (1u << lgepw) - 1u;
   int c = 0;
-  if (s.valid)
-    hash_walk(h, s.l, s.i, s.d0, s.dl,
-              [&](uint32_t loc) { c += hash_is_hit(h, s.l, loc, lgepw, epw_mask); });
+  if (!s.valid) return 0;
+  unsigned cur = 0u;
+  int cw = 0;
+  hash_walk(h, s.l, s.i, s.d0, s.dl, [&](uint32_t loc, int rel) {
+    if (hash_is_hit(h, s.l, loc, lgepw, epw_mask)) {
+      for (; cw < (rel >> 5); ++cw) {
+        mk[cw] = cur;
+        cur = 0u;
+      }
+      cur |= 1u << (rel & 31);
+      ++c;
+    }
+  });
+  for (; cw < words; ++cw) {
+    mk[cw] = cur;
+    cur = 0u;
+  }
   return c;
 }
 
 __global__ __launch_bounds__(HASH_NT) void hash_count_kernel(HashArgs h) {
   __shared__ int sh[16];
   const HashSlot s = hash_slot(h, hash_hit_first_d(h), hash_hit_count_d(h));
-  const int c = hash_count_slice(h, s);
+  const int words = hash_slice_words((long long)h.n / h.B);
+  const long long gid = (long long)blockIdx.x * HASH_NT + threadIdx.x;
+  const int c = hash_count_slice(h, s, h.hmask + gid * words, words);
   int tot = c;   // total of the bucket's lanes (aligned groups of lpj lanes)
   for (int o = 1; o < s.lpj; o <<= 1) tot += __shfl_xor(tot, o, 64);
   const int v = (s.valid && s.sub == 0) ? tot : 0;
@@ -290,9 +318,15 @@ __global__ __launch_bounds__(1024) void hash_scan_kernel(HashArgs h, int nblocks
   if (threadIdx.x == 0) *h.count = carry;
 }
 
+// the hits of each slice, from the count pass's hit mask (no second walk)
 __global__ __launch_bounds__(HASH_NT) void hash_write_kernel(HashArgs h) {
   const HashSlot s = hash_slot(h, hash_hit_first_d(h), hash_hit_count_d(h));
-  const int c = hash_count_slice(h, s);
+  const int words = hash_slice_words((long long)h.n / h.B);
+  const long long gid = (long long)blockIdx.x * HASH_NT + threadIdx.x;
+  const unsigned* __restrict__ mk = h.hmask + gid * words;
+  int c = 0;
+  if (s.valid)
+    for (int w = 0; w < words; ++w) c += __popc(mk[w]);
   // exclusive scan over the bucket's lanes (in window order)
   const int lane = threadIdx.x & 63;
   int x = c;
@@ -301,15 +335,22 @@ __global__ __launch_bounds__(HASH_NT) void hash_write_kernel(HashArgs h) {
     if ((lane & (s.lpj - 1)) >= o) x += y;
   }
   if (!s.valid || c == 0) return;
-  const int lgepw = 5 - __ffs(h.bpe) + 1;
-  const uint32_t epw_mask = (1u << lgepw) - 1u;
   int pos = h.blockoff[blockIdx.x] + h.hitoff[s.jidx] + (x - c);
-  hash_walk(h, s.l, s.i, s.d0, s.dl, [&](uint32_t loc) {
-    if (hash_is_hit(h, s.l, loc, lgepw, epw_mask)) {
-      if (pos < h.capacity) h.hits[pos] = (int)loc;
+  const long long n = h.n;
+  const long long m = n / h.B;
+  const long long J = h.J[(long long)s.l * h.K + s.i];
+  const long long start = ((J * m - (m >> 1)) + n + s.d0) & (n - 1);   // low + d0
+  const uint64_t a = (uint64_t)(uint32_t)h.a_dev[s.l];
+  for (int w = 0; w < words; ++w) {
+    unsigned word = mk[w];
+    while (word) {
+      const int rel = w * 32 + __ffs(word) - 1;
+      word &= word - 1u;
+      const uint64_t p = (uint64_t)((start + rel) & (n - 1));
+      if (pos < h.capacity) h.hits[pos] = (int)((p * a) & ((uint64_t)n - 1));
       ++pos;
     }
-  });
+  }
 }
 
 hipError_t hash_hits_launch(const HashArgs& h, hipStream_t st) {

@@ -136,6 +136,7 @@ struct HashArgs {
   int* hitoff;           // loops_loc x K (local prefix within block)
   int* blocksum;         // blocks
   int* blockoff;         // blocks
+  unsigned* hmask;       // hash_mask_bytes(): one hit mask per walk slice
   int* hits;             // capacity
   int capacity;
   int* count;            // device: total hits (may exceed capacity)
@@ -145,6 +146,7 @@ hipError_t hash_bitmap_launch(const HashArgs& h, hipStream_t st);
 hipError_t hash_vote_launch(const HashArgs& h, hipStream_t st);
 hipError_t hash_hits_launch(const HashArgs& h, hipStream_t st);
 int hash_blocks(int loops_loc, int K, long long m);   // m = n / B
+size_t hash_mask_bytes(int loops_loc, int K, long long m);   // hit masks of the slices
 
 // ---- estimation ---------------------------------------------------------------------------
 // sel (optional, 2 bytes per hit): the loops whose values the real / imaginary lower medians

@@ -276,6 +276,7 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   items.push_back({(void**)&P->hitoff, (size_t)Ll * K * 4});
   items.push_back({(void**)&P->blocksum, (size_t)nhb * 4});
   items.push_back({(void**)&P->blockoff, (size_t)nhb * 4});
+  items.push_back({(void**)&P->hmask, hash_mask_bytes(Ll, K, (long long)prm->n / prm->B)});
   items.push_back({(void**)&P->hits, (size_t)P->cap * 4});
   items.push_back({(void**)&P->count, 16});
   items.push_back({(void**)&P->win, (size_t)P->m * 16});
@@ -389,6 +390,7 @@ HashArgs hash_args(const sfft_plan* P, int* d_count) {
   h.hitoff = P->hitoff;
   h.blocksum = P->blocksum;
   h.blockoff = P->blockoff;
+  h.hmask = P->hmask;
   h.hits = P->hits;
   h.capacity = P->cap;
   h.count = d_count ? d_count : P->count;
@@ -724,10 +726,10 @@ extern "C" sfft_status sfft_execute(sfft_plan* P, const void* d_x, const sfft_dr
   if (!P) return SFFT_ERR_INVALID;
   sfft_status s = sfft_execute_async(P, d_x, dr, P->nodes, P->count);
   if (s != SFFT_OK) return s;
-  // the count and, speculatively, as many nodes as the previous transform had plus 1/8 go
+  // the count and, speculatively, as many nodes as the previous transform had plus 1/32 go
   // back in one round trip; the rest (if the count grew past that) in a second one
   HIPCK(hipMemcpyAsync(P->h_count, P->count, 4, hipMemcpyDeviceToHost, P->st));
-  long long guess = (long long)P->last_count + P->last_count / 8;
+  long long guess = (long long)P->last_count + P->last_count / 32;
   if (guess > P->cap) guess = P->cap;
   if (h_nodes && guess > 0)
     HIPCK(hipMemcpyAsync(h_nodes, P->nodes, (size_t)guess * sizeof(Node), hipMemcpyDeviceToHost,

@@ -41,6 +41,7 @@ struct sfft_plan {
   int* hitoff = nullptr;
   int* blocksum = nullptr;
   int* blockoff = nullptr;
+  unsigned* hmask = nullptr;
   int* hits = nullptr;
   int* count = nullptr;
   double2* win = nullptr;
