@@ -75,13 +75,28 @@ hipError_t fill_u32(void* p, uint32_t v, size_t words, hipStream_t st) {
   return hipGetLastError();
 }
 
+// the per-transform clear in one launch: vote table and bitmaps to 0, the largest approved
+// residue of each location loop to -1 (a grid-stride walk over the three ranges)
+__global__ void hash_clear_kernel(uint32_t* __restrict__ vote, size_t nvote,
+                                  uint32_t* __restrict__ bm, size_t nbm,
+                                  uint32_t* __restrict__ maxfirst, int nmax) {
+  const size_t s = (size_t)gridDim.x * blockDim.x;
+  const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t i = i0; i < nvote; i += s) vote[i] = 0u;
+  for (size_t i = i0; i < nbm; i += s) bm[i] = 0u;
+  if (i0 < (size_t)nmax) maxfirst[i0] = 0xFFFFFFFFu;
+}
+
 hipError_t hash_clear_launch(const HashArgs& h, hipStream_t st) {
   const size_t vote_words = ((size_t)h.n * h.bpe + 31) / 32;
-  hipError_t e = fill_u32(h.vote, 0u, vote_words, st);
-  if (e != hipSuccess) return e;
-  e = fill_u32(h.bitmap, 0u, (size_t)h.loops_loc * h.bm_words, st);
-  if (e != hipSuccess) return e;
-  return fill_u32(h.maxfirst, 0xFFFFFFFFu, (size_t)h.loops_loc, st);
+  const size_t bm_words = (size_t)h.loops_loc * h.bm_words;
+  size_t blocks = (vote_words + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(hash_clear_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
+                     (uint32_t*)h.vote, vote_words, (uint32_t*)h.bitmap, bm_words,
+                     (uint32_t*)h.maxfirst, h.loops_loc);
+  return hipGetLastError();
 }
 
 // approved residue first = J2 * ai mod W of every comb location, as a bitmap per location
