@@ -334,24 +334,50 @@ __global__ __launch_bounds__(CMP_NT) void sel_compact_kernel(int N, int Ksel, in
 // ---- 3. refine over the candidates (one workgroup per segment) ----------------------------
 // The first level scans the candidate list; from then on only the survivors of the previous
 // level (the composite keys under the chosen prefix, usually a handful) are scanned, from LDS
-// (back to the full list if they outgrow RF_CAP).
+// (back to the full list if they outgrow RF_CAP).  With ccnt set the candidates are read
+// where the split left them (block b's at [b*chunk, b*chunk + ccnt[b])) instead of from the
+// compacted list: the ordered mode needs no compaction launch.
 constexpr int RF_CAP = 2048;
 template <typename K>
 __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int emit,
                                                           SelState* __restrict__ st,
                                                           const K* __restrict__ ckey,
                                                           const int* __restrict__ cidx,
-                                                          int* __restrict__ out) {
+                                                          int* __restrict__ out,
+                                                          const int* __restrict__ ccnt,
+                                                          int nchunks, int chunk) {
   __shared__ unsigned h[SEL_BINS];
   __shared__ u128 lst[2][RF_CAP];
   __shared__ int s_len[2];
   __shared__ int sh[16];
   __shared__ int s_digit, s_above, s_exit;
   __shared__ int s_out;
+  __shared__ int s_pre[SEL_MAX_CHUNKS + 1];
   const int tid = threadIdx.x, seg = blockIdx.x;
   const int IB = index_bits(N);
   const int TB = KeyBits<K>::KB + IB;
-  const int nc = st[seg].cand_count;
+  int nc;
+  if (ccnt) {   // exclusive prefix of the per-block candidate counts (nchunks <= 1024)
+    const int v = tid < nchunks ? ccnt[(long long)seg * nchunks + tid] : 0;
+    const int ex = block_excl_scan_s(v, sh);
+    if (tid < nchunks) s_pre[tid] = ex;
+    if (tid == nchunks - 1) s_pre[nchunks] = ex + v;
+    __syncthreads();
+    nc = s_pre[nchunks];
+  } else {
+    nc = st[seg].cand_count;
+  }
+  // storage position of candidate j: the last block b with s_pre[b] <= j holds it
+  auto at = [&](int j) -> long long {
+    if (!ccnt) return j;
+    int lo = 0, hi = nchunks - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_pre[mid] <= j) lo = mid;
+      else hi = mid - 1;
+    }
+    return (long long)lo * chunk + (j - s_pre[lo]);
+  };
   const K* ck = ckey + (long long)seg * N;
   const int* ci = cidx + (long long)seg * N;
   if (tid == 0) s_out = 0;
@@ -375,7 +401,8 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
     const unsigned dmask = (1u << width) - 1u;
     if (cur < 0) {
       for (int j = tid; j < nc; j += 1024) {
-        const u128 c = composite<K>(ck[j], (unsigned)ci[j], N, IB);
+        const long long q = at(j);
+        const u128 c = composite<K>(ck[q], (unsigned)ci[q], N, IB);
         if ((c >> (shift + width)) == prefix) atomicAdd(&h[(unsigned)(c >> shift) & dmask], 1u);
       }
     } else {
@@ -421,7 +448,8 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
       __syncthreads();
       if (cur < 0) {
         for (int j = tid; j < nc; j += 1024) {
-          const u128 c = composite<K>(ck[j], (unsigned)ci[j], N, IB);
+          const long long q = at(j);
+          const u128 c = composite<K>(ck[q], (unsigned)ci[q], N, IB);
           if ((c >> shift) == prefix) {
             const int p = atomicAdd(&s_len[nxt], 1);
             if (p < RF_CAP) lst[nxt][p] = c;
@@ -451,9 +479,10 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
     const int ne = st[seg].out_count;
     for (int base = 0; base < nc; base += 1024) {
       const int j = base + tid;
-      const bool sel = j < nc && composite<K>(ck[j], (unsigned)ci[j], N, IB) >= T;
+      const long long q = j < nc ? at(j) : 0;
+      const bool sel = j < nc && composite<K>(ck[q], (unsigned)ci[q], N, IB) >= T;
       const int po = ne + wave_append(&s_out, sel);
-      if (sel && po < Ksel) o[po] = ci[j];
+      if (sel && po < Ksel) o[po] = ci[q];
     }
   }
 }
@@ -545,12 +574,20 @@ static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ks
   hipLaunchKernelGGL((sel_split_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
                      chunk, nchunks, ordered ? 0 : 1, (const SelState*)S, (K*)w.blk_key,
                      w.blk_idx, w.emit_idx, w.ccnt, w.ecnt);
-  hipLaunchKernelGGL((sel_compact_kernel<K>), grid, dim3(CMP_NT), 0, st, N, Ksel, chunk, nchunks,
-                     ordered ? 0 : 1, S, (const K*)w.blk_key, (const int*)w.blk_idx,
-                     (const int*)w.emit_idx, (const int*)w.ccnt, (const int*)w.ecnt,
-                     (K*)w.cand_key, w.cand_idx, out);
-  hipLaunchKernelGGL((sel_refine_kernel<K>), dim3(segs), dim3(1024), 0, st, N, Ksel,
-                     ordered ? 0 : 1, S, (const K*)w.cand_key, (const int*)w.cand_idx, out);
+  if (ordered) {
+    // the refine reads the candidates from the split's per-block regions
+    hipLaunchKernelGGL((sel_refine_kernel<K>), dim3(segs), dim3(1024), 0, st, N, Ksel, 0, S,
+                       (const K*)w.blk_key, (const int*)w.blk_idx, out, (const int*)w.ccnt,
+                       nchunks, chunk);
+  } else {
+    hipLaunchKernelGGL((sel_compact_kernel<K>), grid, dim3(CMP_NT), 0, st, N, Ksel, chunk,
+                       nchunks, 1, S, (const K*)w.blk_key, (const int*)w.blk_idx,
+                       (const int*)w.emit_idx, (const int*)w.ccnt, (const int*)w.ecnt,
+                       (K*)w.cand_key, w.cand_idx, out);
+    hipLaunchKernelGGL((sel_refine_kernel<K>), dim3(segs), dim3(1024), 0, st, N, Ksel, 1, S,
+                       (const K*)w.cand_key, (const int*)w.cand_idx, out, (const int*)nullptr,
+                       nchunks, chunk);
+  }
   if (ordered) {
     hipLaunchKernelGGL((sel_count_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
                        chunk, nchunks, (const SelState*)S, w.cnt);
