@@ -1,7 +1,8 @@
 // sfft_hash.hip -- hand-written CDNA4 (gfx950) kernels of the sparse-FFT hot path, part 2.
 //
-//  * hash_*_kernel        reverse hash: approved-residue bitmaps, packed per-loop vote masks,
-//                         deterministic hit list (loop, bucket, walk order)
+//  * hash_*_kernel        reverse hash: approved-residue bitmaps, deterministic hit list
+//                         (loop, bucket, walk order); the votes of earlier loops are tested
+//                         directly from their bucket selections (no vote table)
 //  * estimate_kernel      per-location FP64 filter division and stable lower median
 //
 // Floating-point operations that must match the CPU oracle bit for bit are written out
@@ -75,27 +76,24 @@ hipError_t fill_u32(void* p, uint32_t v, size_t words, hipStream_t st) {
   return hipGetLastError();
 }
 
-// the per-transform clear in one launch: vote table and bitmaps to 0, the largest approved
-// residue of each location loop to -1 (a grid-stride walk over the three ranges)
-__global__ void hash_clear_kernel(uint32_t* __restrict__ vote, size_t nvote,
-                                  uint32_t* __restrict__ bm, size_t nbm,
+// the per-transform clear in one launch: approved-residue bitmaps to 0, the largest approved
+// residue of each location loop to -1 (the bucket selection bitmaps are rewritten whole by the
+// cutoff)
+__global__ void hash_clear_kernel(uint32_t* __restrict__ bm, size_t nbm,
                                   uint32_t* __restrict__ maxfirst, int nmax) {
   const size_t s = (size_t)gridDim.x * blockDim.x;
   const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (size_t i = i0; i < nvote; i += s) vote[i] = 0u;
   for (size_t i = i0; i < nbm; i += s) bm[i] = 0u;
   if (i0 < (size_t)nmax) maxfirst[i0] = 0xFFFFFFFFu;
 }
 
 hipError_t hash_clear_launch(const HashArgs& h, hipStream_t st) {
-  const size_t vote_words = ((size_t)h.n * h.bpe + 31) / 32;
   const size_t bm_words = (size_t)h.loops_loc * h.bm_words;
-  size_t blocks = (vote_words + 255) / 256;
+  size_t blocks = (bm_words + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(hash_clear_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
-                     (uint32_t*)h.vote, vote_words, (uint32_t*)h.bitmap, bm_words,
-                     (uint32_t*)h.maxfirst, h.loops_loc);
+                     (uint32_t*)h.bitmap, bm_words, (uint32_t*)h.maxfirst, h.loops_loc);
   return hipGetLastError();
 }
 
@@ -195,12 +193,13 @@ size_t hash_mask_bytes(int loops_loc, int K, long long m) {
 
 // A location is voted at most once per loop (the J windows of a loop are disjoint), so its
 // vote count after loop l is the number of loops 0..l that walked it.  Hence (1) only loops
-// l >= loops_thresh-1 can produce hits, and (2) the hit test of loop l needs only the votes of
-// loops < l (its own is implied by the walk): the last location loop's votes are never read.
-// The vote pass covers loops [0, loops_loc-1), the count and write passes [thresh-1, loops_loc)
-// (loops_thresh >= 1 is checked when the plan is made).
-static int hash_vote_first(const HashArgs& h) { return 0; }
-static int hash_vote_count(const HashArgs& h) { return h.loops_loc - 1; }
+// l >= loops_thresh-1 can produce hits, and (2) the hit test of loop l needs only whether each
+// loop l' < l walked the location (its own walk is implied).  Loop l' walked loc exactly when
+// loc's position p = loc * ai_l' mod n lies in the window of a bucket J that l' selected,
+// J = ((p + m/2) mod n) / m, that window is not cut off by the largest approved residue (the
+// walk's early exit), and p's residue p mod W is approved: three table reads instead of a
+// vote pass over every window and a vote table.  The count and write passes cover the loops
+// [thresh-1, loops_loc) (loops_thresh >= 1 is checked when the plan is made).
 static int hash_hit_count(const HashArgs& h) { return h.loops_loc - (h.loops_thresh - 1); }
 
 // thread -> (location loop l, bucket index i, slice sub of lpj)
@@ -232,49 +231,39 @@ __device__ __forceinline__ HashSlot hash_slot(const HashArgs& h, int l_first, in
   return s;
 }
 
-__global__ __launch_bounds__(HASH_NT) void hash_vote_kernel(HashArgs h, int l_first,
-                                                             int nloops) {
-  const HashSlot s = hash_slot(h, l_first, nloops);
-  if (!s.valid) return;
-  const int lgepw = 5 - __ffs(h.bpe) + 1;  // log2(32 / bpe)
-  const uint32_t epw_mask = (1u << lgepw) - 1u;
-  hash_walk(h, s.l, s.i, s.d0, s.dl, [&](uint32_t loc, int) {
-    atomicOr(&h.vote[loc >> lgepw], (1u << s.l) << ((loc & epw_mask) * (uint32_t)h.bpe));
-  });
-}
-
-hipError_t hash_vote_launch(const HashArgs& h, hipStream_t st) {
-  const int nl = hash_vote_count(h);
-  if (nl <= 0) return hipSuccess;
-  const int nb = hash_blocks(nl, h.K, (long long)h.n / h.B);
-  hipLaunchKernelGGL(hash_vote_kernel, dim3(nb), dim3(HASH_NT), 0, st, h, hash_vote_first(h),
-                     nl);
-  return hipGetLastError();
+// whether location loop lp walked loc: its bucket J was selected, J's window is not cut off
+// by the largest approved residue, and loc's residue is approved
+__device__ __forceinline__ bool hash_walked(const HashArgs& h, int lp, uint32_t loc) {
+  const long long n = h.n;
+  const long long m = n / h.B;
+  const uint64_t p = ((uint64_t)loc * (uint64_t)(uint32_t)h.ai_dev[lp]) & (uint64_t)(n - 1);
+  const uint32_t J = (uint32_t)((((long long)p + (m >> 1)) & (n - 1)) / m);
+  if (!((h.jsel[(size_t)lp * h.jsel_words + (J >> 5)] >> (J & 31)) & 1u)) return false;
+  const long long low = ((long long)J * m - (m >> 1) + n) & (n - 1);
+  if ((int)(low & (h.W - 1)) > h.maxfirst[lp]) return false;
+  const uint32_t r = (uint32_t)(p & (uint64_t)(h.W - 1));
+  return (h.bitmap[(size_t)lp * h.bm_words + (r >> 5)] >> (r & 31)) & 1u;
 }
 
 // a location walked by loop l is a hit of loop l when exactly loops_thresh of the loops 0..l
 // voted for it, loop l included (the loop in which the reference's count reaches the
-// threshold): loops_thresh - 1 of the loops before l
-__device__ __forceinline__ bool hash_is_hit(const HashArgs& h, int l, uint32_t loc, int lgepw,
-                                            uint32_t epw_mask) {
-  const uint32_t word = h.vote[loc >> lgepw];
-  const uint32_t mask = (word >> ((loc & epw_mask) * (uint32_t)h.bpe)) &
-                        (uint32_t)((1ull << h.bpe) - 1ull);
-  return __popc(mask & ((1u << l) - 1u)) + 1 == h.loops_thresh;
+// threshold, innerLoop.cu:809-814): loops_thresh - 1 of the loops before l
+__device__ __forceinline__ bool hash_is_hit(const HashArgs& h, int l, uint32_t loc) {
+  int v = 1;   // loop l's own walk
+  for (int lp = 0; lp < l; ++lp) v += hash_walked(h, lp, loc) ? 1 : 0;
+  return v == h.loops_thresh;
 }
 
 // hit count of a slot's slice, and its hit mask (bit rel of word rel/32 set for a hit at
 // slice position rel) into mk[0 .. words)
 __device__ __forceinline__ int hash_count_slice(const HashArgs& h, const HashSlot& s,
                                                 unsigned* __restrict__ mk, int words) {
-  const int lgepw = 5 - __ffs(h.bpe) + 1;
-  const uint32_t epw_mask = (1u << lgepw) - 1u;
   int c = 0;
   if (!s.valid) return 0;
   unsigned cur = 0u;
   int cw = 0;
   hash_walk(h, s.l, s.i, s.d0, s.dl, [&](uint32_t loc, int rel) {
-    if (hash_is_hit(h, s.l, loc, lgepw, epw_mask)) {
+    if (hash_is_hit(h, s.l, loc)) {
       for (; cw < (rel >> 5); ++cw) {
         mk[cw] = cur;
         cur = 0u;

@@ -117,13 +117,15 @@ struct SelectWork {
 size_t select_work_bytes(int segs, int N);
 void select_work_bind(SelectWork* w, void* base, int segs, int N);
 // out[s*K + j]: the K largest keys of segment s (ties by lower index); ascending indices when
-// `ordered`, otherwise the same set in arbitrary order
+// `ordered`, otherwise the same set in arbitrary order.  bits (optional, ordered mode): the
+// selection as a bitmap, ceil(N/32) words per segment, every word written
 hipError_t select_topk(int is_fp32, const void* keys, long long seg_stride, int N, int K,
-                       int segs, int* out, int ordered, const SelectWork& w, hipStream_t st);
+                       int segs, int* out, int ordered, const SelectWork& w, hipStream_t st,
+                       unsigned* bits = nullptr);
 
-// ---- reverse hash (bitmap of approved residues, votes, deterministic hit list) -----------
+// ---- reverse hash (bitmap of approved residues, deterministic hit list) -----------------
 struct HashArgs {
-  int n, B, K, W, loops_loc, loops_thresh, bpe;
+  int n, B, K, W, loops_loc, loops_thresh;
   const int* J;          // loops_loc x K
   const int* J2;         // nJ2 entries (all comb lists, duplicates allowed)
   int nJ2;
@@ -132,7 +134,8 @@ struct HashArgs {
   unsigned* bitmap;      // loops_loc x words
   int bm_words;
   int* maxfirst;         // loops_loc
-  unsigned* vote;        // n * bpe / 32 words
+  unsigned* jsel;        // loops_loc x jsel_words: the buckets each location loop selected
+  int jsel_words;
   int* hitoff;           // loops_loc x K (local prefix within block)
   int* blocksum;         // blocks
   int* blockoff;         // blocks
@@ -143,7 +146,6 @@ struct HashArgs {
 };
 hipError_t hash_clear_launch(const HashArgs& h, hipStream_t st);
 hipError_t hash_bitmap_launch(const HashArgs& h, hipStream_t st);
-hipError_t hash_vote_launch(const HashArgs& h, hipStream_t st);
 hipError_t hash_hits_launch(const HashArgs& h, hipStream_t st);
 int hash_blocks(int loops_loc, int K, long long m);   // m = n / B
 size_t hash_mask_bytes(int loops_loc, int K, long long m);   // hit masks of the slices

@@ -2,10 +2,11 @@
 // API of include/sfft.h) and the canonical FFT tables shared by every FFT launch.
 //
 // Per transform, on the plan's stream, with no host round trip until the final copy:
-//   clear votes/bitmaps -> comb FFT (gather fused into pass 0, |Y|^2 keys out of the last
+//   clear bitmaps -> comb FFT (gather fused into pass 0, |Y|^2 keys out of the last
 //   pass) -> top-B_thresh of each comb -> approved-residue bitmaps per location loop ->
 //   permute+filter of every loop -> batched bucket FFT (bins_f + keys of location loops) ->
-//   top-B_thresh per location loop -> votes -> deterministic hit list -> estimation.
+//   top-B_thresh per location loop (+ selection bitmaps) -> deterministic hit list ->
+//   estimation.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -237,9 +238,6 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   P->log2W = ilog2_exact(prm->W_RLFC);
   P->m = prm->n / prm->B;
   P->cap = prm->hit_capacity > 0 ? prm->hit_capacity : MAXSIZE_HITS;
-  // vote bits per location: one per location loop whose votes are read (all but the last)
-  P->bpe = 1;
-  while (P->bpe < prm->loops_loc - 1) P->bpe <<= 1;
   const int L = prm->loops, Ll = prm->loops_loc, Rl = prm->RLFC_loops, K = prm->B_thresh;
   const size_t W = prm->W_RLFC, B = prm->B;
   const size_t e = P->fp32 ? 8 : 16, ke = P->fp32 ? 4 : 8;
@@ -272,7 +270,7 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   items.push_back({&selB_base, select_work_bytes(Ll, (int)B)});
   items.push_back({(void**)&P->bitmap, (size_t)Ll * P->bm_words * 4});
   items.push_back({(void**)&P->maxfirst, (size_t)Ll * 4});
-  items.push_back({(void**)&P->vote, ((size_t)prm->n * P->bpe + 31) / 32 * 4});
+  items.push_back({(void**)&P->jsel, (size_t)Ll * ((B + 31) / 32) * 4});
   items.push_back({(void**)&P->hitoff, (size_t)Ll * K * 4});
   items.push_back({(void**)&P->blocksum, (size_t)nhb * 4});
   items.push_back({(void**)&P->blockoff, (size_t)nhb * 4});
@@ -377,7 +375,6 @@ HashArgs hash_args(const sfft_plan* P, int* d_count) {
   h.W = p.W_RLFC;
   h.loops_loc = p.loops_loc;
   h.loops_thresh = p.loops_thresh;
-  h.bpe = P->bpe;
   h.J = P->J;
   h.J2 = P->J2;
   h.nJ2 = p.RLFC_loops * p.B_thresh;
@@ -386,7 +383,8 @@ HashArgs hash_args(const sfft_plan* P, int* d_count) {
   h.bitmap = P->bitmap;
   h.bm_words = P->bm_words;
   h.maxfirst = P->maxfirst;
-  h.vote = P->vote;
+  h.jsel = P->jsel;
+  h.jsel_words = (p.B + 31) / 32;
   h.hitoff = P->hitoff;
   h.blocksum = P->blocksum;
   h.blockoff = P->blockoff;
@@ -401,7 +399,7 @@ HashArgs hash_args(const sfft_plan* P, int* d_count) {
 // for a single transform; a batch spreads its signals over several streams and runs the
 // permute+filter of all signals as one launch).  The stage-boundary events of the profiling
 // mode are recorded by the front and back groups when `marks` is set.
-// front: vote/bitmap clear, streaming pass (option), comb FFT (run_comb), then the comb
+// front: bitmap clear, streaming pass (option), comb FFT (run_comb), then the comb
 // cutoff and the approved bitmaps (run_cut)
 static sfft_status run_comb(sfft_plan* P, const void* d_x, const HashArgs& h, hipStream_t st,
                             bool marks) {
@@ -497,7 +495,7 @@ sfft_status run_bucket(sfft_plan* P, hipStream_t st, bool marks) {
   f.qmax = P->qmax_B;
   HIPCK(fft_run(f, st));
   // ascending bucket lists: the hit list is ordered by (loop, bucket, walk)
-  HIPCK(select_topk(P->fp32, P->bkeys, B, (int)B, K, Ll, P->J, 1, P->selB, st));
+  HIPCK(select_topk(P->fp32, P->bkeys, B, (int)B, K, Ll, P->J, 1, P->selB, st, P->jsel));
   if (marks && P->profile) HIPCK(hipEventRecord(P->ev[5], st));
   return SFFT_OK;
 }
@@ -506,7 +504,6 @@ sfft_status run_bucket(sfft_plan* P, hipStream_t st, bool marks) {
 sfft_status run_locate(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t st,
                        bool marks) {
   const sfft_params& p = P->p;
-  HIPCK(hash_vote_launch(h, st));
   HIPCK(hash_hits_launch(h, st));
   if (marks && P->profile) HIPCK(hipEventRecord(P->ev[6], st));
   HIPCK(estimate_launch(P->fp32, P->hits, h.count, P->cap, P->bins_f, p.loops, p.n, P->d_ai,
@@ -709,7 +706,7 @@ extern "C" sfft_status sfft_plan_stage_times(sfft_plan* P, double* out7) {
   double t[7];
   const sfft_status s = sfft_plan_interval_times(P, t);
   if (s != SFFT_OK) return s;
-  out7[0] = t[0];            // votetable: vote/bitmap clear
+  out7[0] = t[0];            // votetable: bitmap and residue-maximum clear
   out7[1] = t[2];            // restr-loc-index: comb FFT + cutoff + bitmaps
   out7[2] = t[1] + t[3];     // perm+filter (streaming pass + fold, or gathers)
   out7[3] = t[5];            // reverse-hash

@@ -13,7 +13,6 @@ struct sfft_plan {
   hipStream_t st2 = nullptr;    // second chain (permute+filter, bucket FFT, cutoff)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int log2n = 0, log2B = 0, log2W = 0, m = 0;
-  int bpe = 1;
   int cap = MAXSIZE_HITS;
   FftTables tabB, tabW;
   void* ws = nullptr;
@@ -37,7 +36,7 @@ struct sfft_plan {
   unsigned* bitmap = nullptr;
   int bm_words = 0;
   int* maxfirst = nullptr;
-  unsigned* vote = nullptr;
+  unsigned* jsel = nullptr;     // buckets selected per location loop (bitmaps)
   int* hitoff = nullptr;
   int* blocksum = nullptr;
   int* blockoff = nullptr;

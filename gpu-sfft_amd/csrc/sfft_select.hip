@@ -526,7 +526,8 @@ __global__ __launch_bounds__(SEL_NT) void sel_write_kernel(const K* __restrict__
                                                            int nchunks, int Ksel,
                                                            const SelState* __restrict__ st,
                                                            const int* __restrict__ cnt,
-                                                           int* __restrict__ out) {
+                                                           int* __restrict__ out,
+                                                           unsigned* __restrict__ bits) {
   __shared__ int sh[16];
   __shared__ int s_base, s_tot;
   const int seg = blockIdx.y, tid = threadIdx.x;
@@ -545,6 +546,16 @@ __global__ __launch_bounds__(SEL_NT) void sel_write_kernel(const K* __restrict__
   for (int b0 = begin; b0 < end; b0 += SEL_NT) {
     const int i = b0 + tid;
     const bool p = i < end && sel_pred<K>(kp[i], i, N, IB, T);
+    if (bits) {
+      // selection bitmap: each wave owns the two words of its 64 indices (b0 is a multiple of
+      // 64), so every word of the segment is stored once, with plain stores
+      const unsigned long long bal = __ballot(p);
+      const int lane = tid & 63;
+      const int i0 = i - lane;
+      unsigned* bw = bits + (long long)seg * ((N + 31) / 32);
+      if (lane == 0 && i0 < N) bw[i0 >> 5] = (unsigned)bal;
+      if (lane == 32 && i0 + 32 < N) bw[(i0 + 32) >> 5] = (unsigned)(bal >> 32);
+    }
     const int ex = block_excl_scan_s(p ? 1 : 0, sh);
     if (p && base + ex < Ksel) o[base + ex] = i;
     if (tid == SEL_NT - 1) s_tot = ex + (p ? 1 : 0);   // total of this round
@@ -556,7 +567,8 @@ __global__ __launch_bounds__(SEL_NT) void sel_write_kernel(const K* __restrict__
 
 template <typename K>
 static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ksel, int segs,
-                              int* out, int ordered, const SelectWork& w, hipStream_t st) {
+                              int* out, int ordered, const SelectWork& w, hipStream_t st,
+                              unsigned* bits) {
   const int chunk = sel_chunk(N);
   const int nchunks = (N + chunk - 1) / chunk;
   if (nchunks > SEL_MAX_CHUNKS || nchunks > w.max_chunks || segs > w.max_segs)
@@ -592,18 +604,19 @@ static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ks
     hipLaunchKernelGGL((sel_count_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
                        chunk, nchunks, (const SelState*)S, w.cnt);
     hipLaunchKernelGGL((sel_write_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
-                       chunk, nchunks, Ksel, (const SelState*)S, (const int*)w.cnt, out);
+                       chunk, nchunks, Ksel, (const SelState*)S, (const int*)w.cnt, out, bits);
   }
   return hipGetLastError();
 }
 
 hipError_t select_topk(int is_fp32, const void* keys, long long seg_stride, int N, int K,
-                       int segs, int* out, int ordered, const SelectWork& w, hipStream_t st) {
+                       int segs, int* out, int ordered, const SelectWork& w, hipStream_t st,
+                       unsigned* bits) {
   if (is_fp32)
     return select_impl<unsigned int>((const unsigned int*)keys, seg_stride, N, K, segs, out,
-                                     ordered, w, st);
+                                     ordered, w, st, bits);
   return select_impl<unsigned long long>((const unsigned long long*)keys, seg_stride, N, K, segs,
-                                         out, ordered, w, st);
+                                         out, ordered, w, st, bits);
 }
 
 }  // namespace sfft
