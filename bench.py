@@ -59,9 +59,10 @@ def parse():
     ap.add_argument("--design", default="faithful", choices=["faithful", "parallel"],
                     help="filter design mode (setup, outside the timed region): the "
                          "reference's rounding, or the device design without recurrences")
-    ap.add_argument("--graph", action="store_true",
-                    help="replay each transform from a hipGraph instead of enqueuing its "
-                         "kernels eagerly (measured equal at n=2^27: launches keep ahead)")
+    ap.add_argument("--launch", default="graph", choices=["graph", "eager"],
+                    help="replay each transform from a hipGraph captured once (the default: "
+                         "7-12 us per transform faster at n=2^27, k=16000), or enqueue its "
+                         "kernels eagerly")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB before the transforms "
                          "(a known-size kernel that calibrates HBM byte counters)")
@@ -109,7 +110,7 @@ def main():
     front = "gather" if args.front_end == "auto" else args.front_end
     plan = S.Plan(prm, args.precision, front_end=front)
     plan.set_filter(ft, win)
-    if args.graph:
+    if args.launch == "graph":
         plan.set_graph(True)
 
     if args.copy_probe_mib > 0:
@@ -151,7 +152,7 @@ def main():
     stages_ms = {kk: v * 1e3 / prof_steps for kk, v in stage_sum.items()}
     iv_ms = {kk: v * 1e3 / prof_steps for kk, v in iv_sum.items()}
     log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (median "
-        f"{float(np.median(step_ms)):.3f}, {'graph' if args.graph else 'eager'}); "
+        f"{float(np.median(step_ms)):.3f}, {args.launch}); "
         f"stages(ms, eager + events) "
         + " ".join(f"{kk}={v:.3f}" for kk, v in stages_ms.items()))
 
@@ -221,7 +222,7 @@ def main():
         "vs_baseline": round(ms_per_step / PUBLISHED_MS, 6) if args.config == "n27_k16000" and
         not fp32 else None,
         "dtype": "f32" if fp32 else "f64", "data": "synthetic", "front_end": front,
-        "launch": "hipGraph" if args.graph else "eager",
+        "launch": "hipGraph" if args.launch == "graph" else "eager",
         "config": {"workload": f"{args.config}: sfft n=2^{q} k={k} Bcst={Bcst} "
                                f"loops={ll}+{le} thresh={lt} RLFC={rc}x{rl} "
                                f"{'noise ' + str(snr) + 'dB' if snr else 'noiseless'}, "

@@ -101,14 +101,14 @@ struct SelectWork {
   void* state = nullptr;      // per segment: threshold, top digit, counters
   unsigned* hist = nullptr;   // segs * 4096 column sums, then segs * nchunks * 4096 rows
   unsigned* done = nullptr;   // segs (zero between calls)
-  int* cnt = nullptr;         // segs * nchunks
+  int* cnt = nullptr;         // segs * nchunks: selected before each block (ordered mode)
   void* cand_key = nullptr;   // segs * N keys
   int* cand_idx = nullptr;    // segs * N
   int* emit_idx = nullptr;    // segs * N (per-block regions)
   void* blk_key = nullptr;    // segs * N candidate keys in per-block regions
   int* blk_idx = nullptr;     // segs * N
   int* ccnt = nullptr;        // segs * nchunks candidate counts
-  int* ecnt = nullptr;        // segs * nchunks emitted counts
+  int* ecnt = nullptr;        // segs * nchunks counts of digit > d0
   int max_segs = 0;
   int max_chunks = 0;
 };

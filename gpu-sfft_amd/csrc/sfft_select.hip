@@ -10,8 +10,10 @@
 //   3. refine  : one workgroup per segment: MSD radix select on the remaining bits of c over
 //                the candidates only, with early exit once a digit bin fills the remainder;
 //                yields T and (unordered mode) emits the selected candidates.
-//   4. ordered mode only: count + write, an order-preserving compaction of {c >= T} that
-//                produces the ascending index list the reverse hash walks in order.
+//   4. ordered mode only: write, an order-preserving compaction of {c >= T} that produces
+//                the ascending index list the reverse hash walks in order.  The selected
+//                count of each block (its digit > d0 count from the split plus its candidates
+//                >= T) and their exclusive prefix come out of the refine.
 #include "sfft_internal.h"
 
 namespace sfft {
@@ -226,7 +228,8 @@ __global__ __launch_bounds__(PICK_NT) void sel_pick_kernel(int Ksel,
   }
 }
 
-// ---- 2. split: collect digit == d0 (candidates), and digit > d0 (emitted, unordered mode) --
+// ---- 2. split: collect digit == d0 (candidates), count digit > d0 (and emit them, unordered
+// mode) ---------------------------------------------------------------------------------------
 // Block b writes its candidates to [b*chunk, b*chunk + count) of the segment's candidate
 // arrays, its emitted indices to the same range of the emitted array, and the two counts to
 // ccnt/ecnt[seg][b]: no global atomics (the refine pass compacts).
@@ -265,11 +268,9 @@ __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__
       ck[pc] = k;
       ci[pc] = i;
     }
-    if (emit) {
-      const bool ise = ok && d > d0;
-      const int pe = wave_append(&s_ne, ise);
-      if (ise) ce[pe] = i;
-    }
+    const bool ise = ok && d > d0;
+    const int pe = wave_append(&s_ne, ise);
+    if (emit && ise) ce[pe] = i;
   }
   __syncthreads();
   if (tid == 0) {
@@ -336,7 +337,9 @@ __global__ __launch_bounds__(CMP_NT) void sel_compact_kernel(int N, int Ksel, in
 // level (the composite keys under the chosen prefix, usually a handful) are scanned, from LDS
 // (back to the full list if they outgrow RF_CAP).  With ccnt set the candidates are read
 // where the split left them (block b's at [b*chunk, b*chunk + ccnt[b])) instead of from the
-// compacted list: the ordered mode needs no compaction launch.
+// compacted list: the ordered mode needs no compaction launch.  With base set (ordered mode),
+// base[b] = the number of selected elements in blocks < b, from the split's digit > d0 counts
+// ecnt and the candidates >= T of each block: the write pass needs no count pass.
 constexpr int RF_CAP = 2048;
 template <typename K>
 __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int emit,
@@ -345,7 +348,9 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
                                                           const int* __restrict__ cidx,
                                                           int* __restrict__ out,
                                                           const int* __restrict__ ccnt,
-                                                          int nchunks, int chunk) {
+                                                          int nchunks, int chunk,
+                                                          const int* __restrict__ ecnt,
+                                                          int* __restrict__ base) {
   __shared__ unsigned h[SEL_BINS];
   __shared__ u128 lst[2][RF_CAP];
   __shared__ int s_len[2];
@@ -353,6 +358,7 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
   __shared__ int s_digit, s_above, s_exit;
   __shared__ int s_out;
   __shared__ int s_pre[SEL_MAX_CHUNKS + 1];
+  __shared__ int s_bc[SEL_MAX_CHUNKS];
   const int tid = threadIdx.x, seg = blockIdx.x;
   const int IB = index_bits(N);
   const int TB = KeyBits<K>::KB + IB;
@@ -473,12 +479,25 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
     st[seg].thr_hi = (unsigned long long)(T >> 64);
     st[seg].thr_lo = (unsigned long long)T;
   }
+  if (base) {
+    // selected per block: digit > d0 (split) + candidates >= T; then the exclusive prefix
+    for (int b = tid; b < nchunks; b += 1024) s_bc[b] = 0;
+    __syncthreads();
+    for (int j = tid; j < nc; j += 1024) {
+      const long long q = at(j);
+      if (composite<K>(ck[q], (unsigned)ci[q], N, IB) >= T) atomicAdd(&s_bc[(int)(q / chunk)], 1);
+    }
+    __syncthreads();
+    const int v = tid < nchunks ? s_bc[tid] + ecnt[(long long)seg * nchunks + tid] : 0;
+    const int ex = block_excl_scan_s(v, sh);
+    if (tid < nchunks) base[(long long)seg * nchunks + tid] = ex;
+  }
   if (emit) {
     // after the emitted indices (placed by the compaction): the candidates >= T
     int* o = out + (long long)seg * Ksel;
     const int ne = st[seg].out_count;
-    for (int base = 0; base < nc; base += 1024) {
-      const int j = base + tid;
+    for (int j0 = 0; j0 < nc; j0 += 1024) {
+      const int j = j0 + tid;
       const long long q = j < nc ? at(j) : 0;
       const bool sel = j < nc && composite<K>(ck[q], (unsigned)ci[q], N, IB) >= T;
       const int po = ne + wave_append(&s_out, sel);
@@ -494,53 +513,20 @@ __device__ __forceinline__ bool sel_pred(K k, int i, int N, int IB, u128 T) {
 }
 
 template <typename K>
-__global__ __launch_bounds__(SEL_NT) void sel_count_kernel(const K* __restrict__ keys,
-                                                           long long seg_stride, int N, int chunk,
-                                                           int nchunks,
-                                                           const SelState* __restrict__ st,
-                                                           int* __restrict__ cnt) {
-  __shared__ int sh[16];
-  const int seg = blockIdx.y, tid = threadIdx.x;
-  const int IB = index_bits(N);
-  const u128 T = ((u128)st[seg].thr_hi << 64) | (u128)st[seg].thr_lo;
-  const K* kp = keys + (long long)seg * seg_stride;
-  const int begin = blockIdx.x * chunk, end = min(N, begin + chunk);
-  int c = 0;
-  for (int i = begin + tid; i < end; i += SEL_NT) c += sel_pred<K>(kp[i], i, N, IB, T);
-  // block sum
-  const int lane = tid & 63, wid = tid >> 6;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-  if (lane == 0) sh[wid] = c;
-  __syncthreads();
-  if (tid == 0) {
-    int t = 0;
-    for (int w = 0; w < SEL_NT / 64; ++w) t += sh[w];
-    cnt[(long long)seg * nchunks + blockIdx.x] = t;
-  }
-}
-
-template <typename K>
 __global__ __launch_bounds__(SEL_NT) void sel_write_kernel(const K* __restrict__ keys,
                                                            long long seg_stride, int N, int chunk,
                                                            int nchunks, int Ksel,
                                                            const SelState* __restrict__ st,
-                                                           const int* __restrict__ cnt,
+                                                           const int* __restrict__ base0,
                                                            int* __restrict__ out,
                                                            unsigned* __restrict__ bits) {
   __shared__ int sh[16];
-  __shared__ int s_base, s_tot;
+  __shared__ int s_tot;
   const int seg = blockIdx.y, tid = threadIdx.x;
   const int IB = index_bits(N);
   const u128 T = ((u128)st[seg].thr_hi << 64) | (u128)st[seg].thr_lo;
   const K* kp = keys + (long long)seg * seg_stride;
-  if (tid == 0) {
-    int b = 0;
-    for (int c = 0; c < (int)blockIdx.x; ++c) b += cnt[(long long)seg * nchunks + c];
-    s_base = b;
-  }
-  __syncthreads();
-  int base = s_base;
+  int base = base0[(long long)seg * nchunks + blockIdx.x];
   int* o = out + (long long)seg * Ksel;
   const int begin = blockIdx.x * chunk, end = min(N, begin + chunk);
   for (int b0 = begin; b0 < end; b0 += SEL_NT) {
@@ -590,7 +576,7 @@ static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ks
     // the refine reads the candidates from the split's per-block regions
     hipLaunchKernelGGL((sel_refine_kernel<K>), dim3(segs), dim3(1024), 0, st, N, Ksel, 0, S,
                        (const K*)w.blk_key, (const int*)w.blk_idx, out, (const int*)w.ccnt,
-                       nchunks, chunk);
+                       nchunks, chunk, (const int*)w.ecnt, w.cnt);
   } else {
     hipLaunchKernelGGL((sel_compact_kernel<K>), grid, dim3(CMP_NT), 0, st, N, Ksel, chunk,
                        nchunks, 1, S, (const K*)w.blk_key, (const int*)w.blk_idx,
@@ -598,11 +584,9 @@ static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ks
                        (K*)w.cand_key, w.cand_idx, out);
     hipLaunchKernelGGL((sel_refine_kernel<K>), dim3(segs), dim3(1024), 0, st, N, Ksel, 1, S,
                        (const K*)w.cand_key, (const int*)w.cand_idx, out, (const int*)nullptr,
-                       nchunks, chunk);
+                       nchunks, chunk, (const int*)nullptr, (int*)nullptr);
   }
   if (ordered) {
-    hipLaunchKernelGGL((sel_count_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
-                       chunk, nchunks, (const SelState*)S, w.cnt);
     hipLaunchKernelGGL((sel_write_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
                        chunk, nchunks, Ksel, (const SelState*)S, (const int*)w.cnt, out, bits);
   }
