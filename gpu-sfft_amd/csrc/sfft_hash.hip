@@ -133,7 +133,7 @@ hipError_t hash_bitmap_launch(const HashArgs& h, hipStream_t st) {
 // Lanes per bucket window: one per 32 positions (a bitmap word), at most a wave, at least one.
 // m = n/B is a power of two, so the count is a power of two and divides 64.
 __host__ __device__ __forceinline__ int hash_lpj(long long m) {
-  long long l = m / 32;
+  long long l = m / 128;
   if (l < 1) l = 1;
   if (l > 64) l = 64;
   return (int)l;
@@ -237,12 +237,15 @@ __device__ __forceinline__ bool hash_walked(const HashArgs& h, int lp, uint32_t 
   const long long n = h.n;
   const long long m = n / h.B;
   const uint64_t p = ((uint64_t)loc * (uint64_t)(uint32_t)h.ai_dev[lp]) & (uint64_t)(n - 1);
-  const uint32_t J = (uint32_t)((((long long)p + (m >> 1)) & (n - 1)) / m);
-  if (!((h.jsel[(size_t)lp * h.jsel_words + (J >> 5)] >> (J & 31)) & 1u)) return false;
+  const int lm = 63 - __clzll(m);   // m = n / B, both powers of two
+  const uint32_t J = (uint32_t)((((long long)p + (m >> 1)) & (n - 1)) >> lm);
   const long long low = ((long long)J * m - (m >> 1) + n) & (n - 1);
-  if ((int)(low & (h.W - 1)) > h.maxfirst[lp]) return false;
   const uint32_t r = (uint32_t)(p & (uint64_t)(h.W - 1));
-  return (h.bitmap[(size_t)lp * h.bm_words + (r >> 5)] >> (r & 31)) & 1u;
+  // the three table reads are independent: issued together, tested together
+  const unsigned js = h.jsel[(size_t)lp * h.jsel_words + (J >> 5)];
+  const int mf = h.maxfirst[lp];
+  const unsigned bw = h.bitmap[(size_t)lp * h.bm_words + (r >> 5)];
+  return (((js >> (J & 31)) & (bw >> (r & 31))) & 1u) && (int)(low & (h.W - 1)) <= mf;
 }
 
 // a location walked by loop l is a hit of loop l when exactly loops_thresh of the loops 0..l
@@ -293,37 +296,15 @@ __global__ __launch_bounds__(HASH_NT) void hash_count_kernel(HashArgs h) {
   if (threadIdx.x == HASH_NT - 1) h.blocksum[blockIdx.x] = ex + v;
 }
 
-// exclusive scan of the per-block hit totals: 1024 threads x 8 consecutive blocks per round
-__global__ __launch_bounds__(1024) void hash_scan_kernel(HashArgs h, int nblocks) {
-  __shared__ int sh[16];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < nblocks; base += 8192) {
-    const int i0 = base + threadIdx.x * 8;
-    int v[8], s = 0;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      v[u] = (i0 + u < nblocks) ? h.blocksum[i0 + u] : 0;
-      s += v[u];
-    }
-    const int ex = block_excl_scan(s, sh);
-    const int c0 = carry;
-    int run = c0 + ex;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (i0 + u < nblocks) h.blockoff[i0 + u] = run;
-      run += v[u];
-    }
-    __syncthreads();
-    if (threadIdx.x == 1023) carry = run;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *h.count = carry;
-}
-
-// the hits of each slice, from the count pass's hit mask (no second walk)
+// the hits of each slice, from the count pass's hit mask (no second walk).  Each block first
+// sums the hit totals of the blocks before it (a few KB, read by every block in parallel: no
+// separate scan launch); the last block stores the total hit count.
 __global__ __launch_bounds__(HASH_NT) void hash_write_kernel(HashArgs h) {
+  __shared__ int sh[16];
+  int pre = 0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += HASH_NT) pre += h.blocksum[b];
+  const int boff = block_sum(pre, sh);
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *h.count = boff + h.blocksum[blockIdx.x];
   const HashSlot s = hash_slot(h, hash_hit_first_d(h), hash_hit_count_d(h));
   const int words = hash_slice_words((long long)h.n / h.B);
   const long long gid = (long long)blockIdx.x * HASH_NT + threadIdx.x;
@@ -339,7 +320,7 @@ __global__ __launch_bounds__(HASH_NT) void hash_write_kernel(HashArgs h) {
     if ((lane & (s.lpj - 1)) >= o) x += y;
   }
   if (!s.valid || c == 0) return;
-  int pos = h.blockoff[blockIdx.x] + h.hitoff[s.jidx] + (x - c);
+  int pos = boff + h.hitoff[s.jidx] + (x - c);
   const long long n = h.n;
   const long long m = n / h.B;
   const long long J = h.J[(long long)s.l * h.K + s.i];
@@ -360,7 +341,6 @@ __global__ __launch_bounds__(HASH_NT) void hash_write_kernel(HashArgs h) {
 hipError_t hash_hits_launch(const HashArgs& h, hipStream_t st) {
   const int nb = hash_blocks(hash_hit_count(h), h.K, (long long)h.n / h.B);
   hipLaunchKernelGGL(hash_count_kernel, dim3(nb), dim3(HASH_NT), 0, st, h);
-  hipLaunchKernelGGL(hash_scan_kernel, dim3(1), dim3(1024), 0, st, h, nb);
   hipLaunchKernelGGL(hash_write_kernel, dim3(nb), dim3(HASH_NT), 0, st, h);
   return hipGetLastError();
 }

@@ -137,8 +137,7 @@ struct HashArgs {
   unsigned* jsel;        // loops_loc x jsel_words: the buckets each location loop selected
   int jsel_words;
   int* hitoff;           // loops_loc x K (local prefix within block)
-  int* blocksum;         // blocks
-  int* blockoff;         // blocks
+  int* blocksum;         // hit total of each count/write block
   unsigned* hmask;       // hash_mask_bytes(): one hit mask per walk slice
   int* hits;             // capacity
   int capacity;

@@ -273,7 +273,6 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   items.push_back({(void**)&P->jsel, (size_t)Ll * ((B + 31) / 32) * 4});
   items.push_back({(void**)&P->hitoff, (size_t)Ll * K * 4});
   items.push_back({(void**)&P->blocksum, (size_t)nhb * 4});
-  items.push_back({(void**)&P->blockoff, (size_t)nhb * 4});
   items.push_back({(void**)&P->hmask, hash_mask_bytes(Ll, K, (long long)prm->n / prm->B)});
   items.push_back({(void**)&P->hits, (size_t)P->cap * 4});
   items.push_back({(void**)&P->count, 16});
@@ -387,7 +386,6 @@ HashArgs hash_args(const sfft_plan* P, int* d_count) {
   h.jsel_words = (p.B + 31) / 32;
   h.hitoff = P->hitoff;
   h.blocksum = P->blocksum;
-  h.blockoff = P->blockoff;
   h.hmask = P->hmask;
   h.hits = P->hits;
   h.capacity = P->cap;

@@ -39,7 +39,6 @@ struct sfft_plan {
   unsigned* jsel = nullptr;     // buckets selected per location loop (bitmaps)
   int* hitoff = nullptr;
   int* blocksum = nullptr;
-  int* blockoff = nullptr;
   unsigned* hmask = nullptr;
   int* hits = nullptr;
   int* count = nullptr;
