@@ -353,40 +353,52 @@ __device__ __forceinline__ double2 to_d2(float2 v) {
   return make_double2((double)v.x, (double)v.y);
 }
 
-template <typename Real, int MAXL>
+// One thread per hit.  EXACT: the loop count is the template constant MAXL (every address
+// first, then every load, so the 2*MAXL reads of a hit are in flight together); otherwise
+// loops <= MAXL at run time.  The arithmetic per loop is the same either way.
+template <typename Real, int MAXL, bool EXACT>
 __global__ __launch_bounds__(256) void estimate_kernel(
     const int* __restrict__ hits, const int* __restrict__ count, int capacity,
-    const typename CT<Real>::T2* __restrict__ bins, int loops, uint32_t n,
+    const typename CT<Real>::T2* __restrict__ bins, int loops_rt, uint32_t n,
     const int* __restrict__ ai_dev, int B, const double2* __restrict__ win,
     Node* __restrict__ out, uint8_t* __restrict__ sel) {
+  const int loops = EXACT ? MAXL : loops_rt;
   const int total = min(*count, capacity);
   const long long m = (long long)(n / (uint32_t)B);
+  const int lm = 63 - __clzll(m);   // m = n / B, both powers of two
   const long long nn = n;
   const int med = (loops - 1) / 2;
   for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < total; h += gridDim.x * blockDim.x) {
     const uint32_t I = (uint32_t)hits[h];
+    long long bidx[MAXL], widx[MAXL];
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l) {
+      const int lc = (EXACT || l < loops) ? l : 0;   // a valid address past the loop count
+      const long long p = (long long)(((uint64_t)(uint32_t)ai_dev[lc] * I) & (uint64_t)(n - 1));
+      long long hashed = p >> lm;
+      long long dist = p & (m - 1);
+      if (dist > m / 2) {
+        hashed = (hashed + 1) & (B - 1);
+        dist -= m;
+      }
+      dist = (nn - dist) & (nn - 1);
+      widx[l] = (dist + m / 2) & (nn - 1);
+      bidx[l] = (long long)lc * B + hashed;
+    }
+    double2 F[MAXL], bb[MAXL];
+#pragma unroll
+    for (int l = 0; l < MAXL; ++l) {
+      F[l] = win[widx[l]];
+      bb[l] = to_d2(bins[bidx[l]]);
+    }
     double vr[MAXL], vi[MAXL];
 #pragma unroll
     for (int l = 0; l < MAXL; ++l) {
-      vr[l] = 0.0;
-      vi[l] = 0.0;
-      if (l < loops) {
-        const long long p = (long long)(((uint64_t)(uint32_t)ai_dev[l] * I) & (uint64_t)(n - 1));
-        long long hashed = p / m;
-        long long dist = p & (m - 1);
-        if (dist > m / 2) {
-          hashed = (hashed + 1) & (B - 1);
-          dist -= m;
-        }
-        dist = (nn - dist) & (nn - 1);
-        const double2 F = win[(dist + m / 2) & (nn - 1)];
-        const double2 bb = to_d2(bins[(long long)l * B + hashed]);
-        const double abs2 = fma(F.x, F.x, F.y * F.y);
-        const double mr = fma(bb.x, F.x, bb.y * F.y);
-        const double mi = fma(bb.y, F.x, -(bb.x * F.y));
-        vr[l] = mr / abs2;
-        vi[l] = mi / abs2;
-      }
+      const double abs2 = fma(F[l].x, F[l].x, F[l].y * F[l].y);
+      const double mr = fma(bb[l].x, F[l].x, bb[l].y * F[l].y);
+      const double mi = fma(bb[l].y, F[l].x, -(bb[l].x * F[l].y));
+      vr[l] = mr / abs2;
+      vi[l] = mi / abs2;
     }
     // stable rank selection: position of j in an insertion sort = #(v_i < v_j) + #(i<j, v_i==v_j)
     double rr = 0.0, ri = 0.0;
@@ -424,22 +436,34 @@ __global__ __launch_bounds__(256) void estimate_kernel(
   }
 }
 
+template <typename Real, int L, bool EXACT>
+static void estimate_go(const int* hits, const int* count, int capacity, const void* bins,
+                        int loops, int n, const int* ai_dev, int B, const double2* win,
+                        Node* out, uint8_t* sel, hipStream_t st) {
+  using T2 = typename CT<Real>::T2;
+  hipLaunchKernelGGL((estimate_kernel<Real, L, EXACT>), dim3(512), dim3(256), 0, st, hits, count,
+                     capacity, (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out, sel);
+}
+
 template <typename Real>
 static hipError_t estimate_dispatch(const int* hits, const int* count, int capacity,
                                     const void* bins, int loops, int n, const int* ai_dev, int B,
                                     const double2* win, Node* out, uint8_t* sel,
                                     hipStream_t st) {
-  using T2 = typename CT<Real>::T2;
-  const dim3 grid(512), block(256);
-  if (loops <= 8)
-    hipLaunchKernelGGL((estimate_kernel<Real, 8>), grid, block, 0, st, hits, count, capacity,
-                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out, sel);
-  else if (loops <= 16)
-    hipLaunchKernelGGL((estimate_kernel<Real, 16>), grid, block, 0, st, hits, count, capacity,
-                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out, sel);
-  else
-    hipLaunchKernelGGL((estimate_kernel<Real, 32>), grid, block, 0, st, hits, count, capacity,
-                       (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out, sel);
+#define SFFT_EST(L)                                                                         \
+  case L:                                                                                   \
+    estimate_go<Real, L, true>(hits, count, capacity, bins, loops, n, ai_dev, B, win, out, \
+                               sel, st);                                                    \
+    break;
+  switch (loops) {
+    SFFT_EST(1) SFFT_EST(2) SFFT_EST(3) SFFT_EST(4) SFFT_EST(5) SFFT_EST(6) SFFT_EST(7)
+    SFFT_EST(8) SFFT_EST(9) SFFT_EST(10) SFFT_EST(11) SFFT_EST(12) SFFT_EST(13) SFFT_EST(14)
+    SFFT_EST(15) SFFT_EST(16)
+    default:
+      estimate_go<Real, 32, false>(hits, count, capacity, bins, loops, n, ai_dev, B, win, out,
+                                   sel, st);
+  }
+#undef SFFT_EST
   return hipGetLastError();
 }
 
