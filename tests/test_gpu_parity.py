@@ -116,6 +116,22 @@ def test_transform_stream_front_end_bitexact(S, cfg, fold_mode):
     _assert_transform_equal(prm, plan, ref, nodes, cnt)
 
 
+# (loops_loc, loops_thresh, loops_est): one loop; the threshold reached at the first vote
+# (every walked location not walked by an earlier loop); hit tests over two, three and four
+# earlier loops; total loop counts 1, 5, 8, 10, 16 (the estimation kernel's compile-time loop
+# counts) and 17, 32 (its run-time loop count, up to the largest the plan accepts)
+LOOP_VARIANTS = [(1, 1, 0), (3, 1, 2), (4, 3, 4), (3, 3, 7), (5, 4, 11), (6, 4, 11), (8, 5, 24)]
+
+
+@pytest.mark.parametrize("loops", LOOP_VARIANTS, ids=lambda v: "loc%d_thr%d_est%d" % v)
+def test_transform_loop_counts_bitexact(S, loops):
+    cfg = (14, 20, 1.0) + tuple(loops) + (4, 2)
+    assert params(cfg)["assert_ok"] == 1
+    prm, plan, ref, nodes, cnt, _ = _run_both(S, cfg, seed=sum(loops) * 7 + 1)
+    assert prm["loops"] == loops[0] + loops[2]
+    _assert_transform_equal(prm, plan, ref, nodes, cnt)
+
+
 def test_transform_noisy_bitexact(S):
     prm, plan, ref, nodes, cnt, _ = _run_both(S, SMALL_CONFIGS[2], seed=11, snr_db=20.0)
     _assert_transform_equal(prm, plan, ref, nodes, cnt)
