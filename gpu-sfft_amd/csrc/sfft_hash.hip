@@ -355,21 +355,31 @@ __device__ __forceinline__ double2 to_d2(float2 v) {
 
 // One thread per hit.  EXACT: the loop count is the template constant MAXL (every address
 // first, then every load, so the 2*MAXL reads of a hit are in flight together); otherwise
-// loops <= MAXL at run time.  The arithmetic per loop is the same either way.
+// loops <= MAXL at run time.  The arithmetic per loop is the same either way.  A block's
+// nodes are staged in LDS and stored as consecutive 8-byte words (coalesced, also when `out`
+// is pinned host memory written over the bus); count_copy (optional) receives the count.
+constexpr int EST_NT = 256;
+constexpr int EST_GRID = 512;   // blocks of the estimation launch (grid-stride over the hits)
 template <typename Real, int MAXL, bool EXACT>
-__global__ __launch_bounds__(256) void estimate_kernel(
+__global__ __launch_bounds__(EST_NT) void estimate_kernel(
     const int* __restrict__ hits, const int* __restrict__ count, int capacity,
     const typename CT<Real>::T2* __restrict__ bins, int loops_rt, uint32_t n,
     const int* __restrict__ ai_dev, int B, const double2* __restrict__ win,
-    Node* __restrict__ out, uint8_t* __restrict__ sel) {
+    Node* __restrict__ out, uint8_t* __restrict__ sel, int* __restrict__ count_copy) {
+  static_assert(sizeof(Node) == 24, "Node: key, 4 bytes of padding, two doubles");
+  __shared__ unsigned long long stage[EST_NT * 3];
   const int loops = EXACT ? MAXL : loops_rt;
-  const int total = min(*count, capacity);
+  const int cnt_all = *count;
+  const int total = min(cnt_all, capacity);
+  if (count_copy && blockIdx.x == 0 && threadIdx.x == 0) *count_copy = cnt_all;
   const long long m = (long long)(n / (uint32_t)B);
   const int lm = 63 - __clzll(m);   // m = n / B, both powers of two
   const long long nn = n;
   const int med = (loops - 1) / 2;
-  for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < total; h += gridDim.x * blockDim.x) {
-    const uint32_t I = (uint32_t)hits[h];
+  for (int h0 = blockIdx.x * EST_NT; h0 < total; h0 += gridDim.x * EST_NT) {
+    const int h = h0 + (int)threadIdx.x;
+    const bool valid = h < total;
+    const uint32_t I = (uint32_t)hits[valid ? h : h0];
     long long bidx[MAXL], widx[MAXL];
 #pragma unroll
     for (int l = 0; l < MAXL; ++l) {
@@ -424,36 +434,41 @@ __global__ __launch_bounds__(256) void estimate_kernel(
         }
       }
     }
-    Node nd;
-    nd.key = I;
-    nd.value.x = rr;
-    nd.value.y = ri;
-    out[h] = nd;
-    if (sel) {
+    stage[3 * threadIdx.x] = (unsigned long long)I;   // key, then 4 zero bytes of padding
+    stage[3 * threadIdx.x + 1] = (unsigned long long)__double_as_longlong(rr);
+    stage[3 * threadIdx.x + 2] = (unsigned long long)__double_as_longlong(ri);
+    if (sel && valid) {
       sel[2 * h] = (uint8_t)sr;
       sel[2 * h + 1] = (uint8_t)si;
     }
+    __syncthreads();
+    const int words = 3 * min(EST_NT, total - h0);
+    unsigned long long* o = reinterpret_cast<unsigned long long*>(out + h0);
+    for (int j = threadIdx.x; j < words; j += EST_NT) o[j] = stage[j];
+    __syncthreads();
   }
 }
 
 template <typename Real, int L, bool EXACT>
 static void estimate_go(const int* hits, const int* count, int capacity, const void* bins,
                         int loops, int n, const int* ai_dev, int B, const double2* win,
-                        Node* out, uint8_t* sel, hipStream_t st) {
+                        Node* out, uint8_t* sel, int* count_copy, hipStream_t st) {
   using T2 = typename CT<Real>::T2;
-  hipLaunchKernelGGL((estimate_kernel<Real, L, EXACT>), dim3(512), dim3(256), 0, st, hits, count,
-                     capacity, (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out, sel);
+  hipLaunchKernelGGL((estimate_kernel<Real, L, EXACT>), dim3(EST_GRID), dim3(EST_NT), 0, st,
+                     hits,
+                     count, capacity, (const T2*)bins, loops, (uint32_t)n, ai_dev, B, win, out,
+                     sel, count_copy);
 }
 
 template <typename Real>
 static hipError_t estimate_dispatch(const int* hits, const int* count, int capacity,
                                     const void* bins, int loops, int n, const int* ai_dev, int B,
                                     const double2* win, Node* out, uint8_t* sel,
-                                    hipStream_t st) {
+                                    int* count_copy, hipStream_t st) {
 #define SFFT_EST(L)                                                                         \
   case L:                                                                                   \
     estimate_go<Real, L, true>(hits, count, capacity, bins, loops, n, ai_dev, B, win, out, \
-                               sel, st);                                                    \
+                               sel, count_copy, st);                                        \
     break;
   switch (loops) {
     SFFT_EST(1) SFFT_EST(2) SFFT_EST(3) SFFT_EST(4) SFFT_EST(5) SFFT_EST(6) SFFT_EST(7)
@@ -461,7 +476,7 @@ static hipError_t estimate_dispatch(const int* hits, const int* count, int capac
     SFFT_EST(15) SFFT_EST(16)
     default:
       estimate_go<Real, 32, false>(hits, count, capacity, bins, loops, n, ai_dev, B, win, out,
-                                   sel, st);
+                                   sel, count_copy, st);
   }
 #undef SFFT_EST
   return hipGetLastError();
@@ -470,12 +485,12 @@ static hipError_t estimate_dispatch(const int* hits, const int* count, int capac
 hipError_t estimate_launch(int is_fp32, const int* hits, const int* count, int capacity,
                            const void* bins_f, int loops, int n, const int* ai_dev, int B,
                            const double2* filter_win, Node* out, uint8_t* sel,
-                           hipStream_t st) {
+                           hipStream_t st, int* count_copy) {
   if (is_fp32)
     return estimate_dispatch<float>(hits, count, capacity, bins_f, loops, n, ai_dev, B,
-                                    filter_win, out, sel, st);
+                                    filter_win, out, sel, count_copy, st);
   return estimate_dispatch<double>(hits, count, capacity, bins_f, loops, n, ai_dev, B,
-                                   filter_win, out, sel, st);
+                                   filter_win, out, sel, count_copy, st);
 }
 
 }  // namespace sfft

@@ -151,11 +151,13 @@ size_t hash_mask_bytes(int loops_loc, int K, long long m);   // hit masks of the
 
 // ---- estimation ---------------------------------------------------------------------------
 // sel (optional, 2 bytes per hit): the loops whose values the real / imaginary lower medians
-// returned (stable-rank order)
+// returned (stable-rank order).  out may be device memory or the device view of pinned host
+// memory (stored in coalesced 8-byte words either way); count_copy (optional, the same
+// kinds of memory) receives *count.
 hipError_t estimate_launch(int is_fp32, const int* hits, const int* count, int capacity,
                            const void* bins_f, int loops, int n, const int* ai_dev, int B,
                            const double2* filter_win, Node* out, uint8_t* sel,
-                           hipStream_t st);
+                           hipStream_t st, int* count_copy = nullptr);
 
 // ---- value map and its adjoint (FP64) -----------------------------------------------------
 // values[h] = Re v_{sel[2h]}(hits[h]) + i Im v_{sel[2h+1]}(hits[h]) from the bucket spectra

@@ -305,6 +305,7 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
       hipHostMalloc((void**)&P->h_draws, (size_t)(2 * L + Rl) * 4, hipHostMallocDefault) !=
           hipSuccess ||
       hipHostMalloc((void**)&P->h_count, 16, hipHostMallocDefault) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&P->h_count_dev, P->h_count, 0) != hipSuccess ||
       hipEventCreateWithFlags(&P->up_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming) != hipSuccess ||
@@ -505,7 +506,7 @@ sfft_status run_locate(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream
   HIPCK(hash_hits_launch(h, st));
   if (marks && P->profile) HIPCK(hipEventRecord(P->ev[6], st));
   HIPCK(estimate_launch(P->fp32, P->hits, h.count, P->cap, P->bins_f, p.loops, p.n, P->d_ai,
-                        p.B, P->win, d_nodes ? d_nodes : P->nodes, P->sel, st));
+                        p.B, P->win, d_nodes ? d_nodes : P->nodes, P->sel, st, P->cnt_copy));
   if (marks && P->profile) HIPCK(hipEventRecord(P->ev[7], st));
   return SFFT_OK;
 }
@@ -546,15 +547,24 @@ static void drop_plan_graph(sfft_plan* P) {
   P->graph = nullptr;
 }
 
-extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const sfft_draws* dr,
-                                          Node* d_nodes, int* d_count) {
+// One transform enqueued on the plan stream: nodes to d_nodes (device memory, or the device
+// view of pinned host memory; nullptr: the plan's buffer), the count to d_count (nullptr: the
+// plan's) and, when cnt_copy is set, also to *cnt_copy.
+static sfft_status enqueue(sfft_plan* P, const void* d_x, const sfft_draws* dr, Node* d_nodes,
+                           int* d_count, int* cnt_copy) {
   if (!P || !d_x) return SFFT_ERR_INVALID;
   if (!P->has_filter) return SFFT_ERR_NO_FILTER;
   sfft_status v = upload_draws(P, dr);
   if (v != SFFT_OK) return v;
   const HashArgs h = hash_args(P, d_count);
+  P->cnt_copy = cnt_copy;   // read by run_locate while this transform is enqueued / captured
+  struct Reset {
+    sfft_plan* P;
+    ~Reset() { P->cnt_copy = nullptr; }
+  } reset{P};
   if (!P->use_graph || P->profile) return run_all(P, d_x, h, d_nodes);   // eager
-  if (P->gexec && (P->g_x != d_x || P->g_nodes != d_nodes || P->g_count != d_count))
+  if (P->gexec && (P->g_x != d_x || P->g_nodes != d_nodes || P->g_count != d_count ||
+                   P->g_cnt_copy != cnt_copy))
     drop_plan_graph(P);
   if (!P->gexec) {
     HIPCK(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
@@ -573,9 +583,35 @@ extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const s
     P->g_x = d_x;
     P->g_nodes = d_nodes;
     P->g_count = d_count;
+    P->g_cnt_copy = cnt_copy;
   }
   HIPCK(hipGraphLaunch(P->gexec, P->st));
   return SFFT_OK;
+}
+
+extern "C" sfft_status sfft_execute_async(sfft_plan* P, const void* d_x, const sfft_draws* dr,
+                                          Node* d_nodes, int* d_count) {
+  return enqueue(P, d_x, dr, d_nodes, d_count, nullptr);
+}
+
+// The device view of h when h is pinned host memory with room for `bytes` from h on, else
+// nullptr (pageable memory, or too short an allocation).  Asked on every call: an address
+// freed and reused for pageable memory must never be written through a stale view.
+static void* pinned_view(void* h, size_t bytes) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, h) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (at.type != hipMemoryTypeHost || !at.devicePointer) return nullptr;
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)at.devicePointer) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  const size_t off = (size_t)((char*)at.devicePointer - (char*)base);
+  return (off <= size && size - off >= bytes) ? at.devicePointer : nullptr;
 }
 
 // Graph mode: the transform's launch sequence is captured once and replayed (the draws are
@@ -719,7 +755,19 @@ extern "C" sfft_status sfft_plan_stage_times(sfft_plan* P, double* out7) {
 extern "C" sfft_status sfft_execute(sfft_plan* P, const void* d_x, const sfft_draws* dr,
                                     Node* h_nodes, int* h_count) {
   if (!P) return SFFT_ERR_INVALID;
-  sfft_status s = sfft_execute_async(P, d_x, dr, P->nodes, P->count);
+  // pinned h_nodes: the estimation kernel stores the nodes and the count straight into host
+  // memory (no copies after the transform)
+  Node* zc = h_nodes ? (Node*)pinned_view(h_nodes, (size_t)P->cap * sizeof(Node)) : nullptr;
+  if (zc) {
+    sfft_status s = enqueue(P, d_x, dr, zc, P->count, P->h_count_dev);
+    if (s != SFFT_OK) return s;
+    HIPCK(hipStreamSynchronize(P->st));
+    const int cnt = *(volatile int*)P->h_count;
+    P->last_count = cnt;
+    if (h_count) *h_count = cnt;
+    return cnt > P->cap ? SFFT_ERR_HIT_OVERFLOW : SFFT_OK;
+  }
+  sfft_status s = enqueue(P, d_x, dr, P->nodes, P->count, nullptr);
   if (s != SFFT_OK) return s;
   // the count and, speculatively, as many nodes as the previous transform had plus 1/32 go
   // back in one round trip; the rest (if the count grew past that) in a second one

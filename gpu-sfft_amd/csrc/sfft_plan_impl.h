@@ -57,6 +57,10 @@ struct sfft_plan {
   // pinned host staging: draws (a, ai, offsets packed like d_a..d_offs) and the hit count
   int* h_draws = nullptr;
   int* h_count = nullptr;
+  int* h_count_dev = nullptr;   // device view of h_count
+  // where the estimation kernel also stores the hit count (nullptr: nowhere); part of the
+  // captured graph, like the node and count pointers
+  int* cnt_copy = nullptr;
   hipEvent_t up_ev = nullptr;   // last upload out of h_draws
   int up_pending = 0;
   // optional stage timing (hipEvents at the reference's stage boundaries)
@@ -70,6 +74,7 @@ struct sfft_plan {
   const void* g_x = nullptr;
   Node* g_nodes = nullptr;
   int* g_count = nullptr;
+  int* g_cnt_copy = nullptr;
   hipEvent_t ev[9] = {nullptr, nullptr, nullptr, nullptr, nullptr,
                       nullptr, nullptr, nullptr, nullptr};
 };

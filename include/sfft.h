@@ -187,8 +187,11 @@ sfft_status sfft_plan_set_filter(sfft_plan *plan, const void *d_filter_time,
  * int).  Asynchronous on the plan's stream. */
 sfft_status sfft_execute_async(sfft_plan *plan, const void *d_x, const sfft_draws *draws,
                                Node *d_nodes, int *d_count);
-/* Same, then copies the count and nodes to host memory (h_nodes has hit_capacity entries)
- * and synchronises the stream.  Returns SFFT_ERR_HIT_OVERFLOW when the hits exceed capacity. */
+/* Same, with the count and nodes in host memory (h_nodes has hit_capacity entries), then
+ * synchronises the stream.  When h_nodes is pinned host memory (hipHostMalloc, or memory the
+ * HIP runtime reports as pinned) the estimation kernel stores the nodes and the count into it
+ * directly; otherwise they are copied after the transform.  Returns SFFT_ERR_HIT_OVERFLOW
+ * when the hits exceed capacity. */
 sfft_status sfft_execute(sfft_plan *plan, const void *d_x, const sfft_draws *draws,
                          Node *h_nodes, int *h_count);
 

@@ -132,6 +132,34 @@ def test_transform_loop_counts_bitexact(S, loops):
     _assert_transform_equal(prm, plan, ref, nodes, cnt)
 
 
+def test_execute_pinned_and_pageable_outputs_agree(S):
+    """sfft_execute stores the nodes and the count straight into pinned host memory (the
+    binding's buffer) and copies them into pageable memory (a plain numpy array): both give
+    the oracle's nodes, on repeated calls (the pageable path's copy size follows the last
+    count)."""
+    import ctypes as C
+    prm, plan, ref, nodes, cnt, _ = _run_both(S, SMALL_CONFIGS[2], seed=77)
+    _assert_transform_equal(prm, plan, ref, nodes, cnt)
+    n = prm["n"]
+    x, _, _ = sparse_signal(n, prm["k"], 77)
+    a, ai, offs = draws(n, prm["loops"], prm["RLFC_loops"], prm["W_RLFC"], 77 + 7)
+    dx = _dev(x)
+    for rep in range(3):
+        host = np.zeros(plan.capacity, dtype=S.NODE_DTYPE)      # pageable
+        dr = plan._draws(a, ai, offs)
+        got = C.c_int(-1)
+        st = S.lib.sfft_execute(plan._h, C.c_void_p(dx.data_ptr()), C.byref(dr),
+                                host.ctypes.data_as(C.c_void_p), C.byref(got))
+        assert st == 0 and got.value == ref["I_F"]
+        out = host[:got.value]
+        assert np.array_equal(out["key"], ref["nodes"]["key"])
+        assert np.array_equal(bits(out["re"]), bits(ref["nodes"]["re"]))
+        assert np.array_equal(bits(out["im"]), bits(ref["nodes"]["im"]))
+        again, c2 = plan.execute(dx, a, ai, offs)               # pinned, between the calls
+        assert c2 == ref["I_F"] and np.array_equal(again["key"], ref["nodes"]["key"])
+        assert np.array_equal(bits(again["re"]), bits(ref["nodes"]["re"]))
+
+
 def test_transform_noisy_bitexact(S):
     prm, plan, ref, nodes, cnt, _ = _run_both(S, SMALL_CONFIGS[2], seed=11, snr_db=20.0)
     _assert_transform_equal(prm, plan, ref, nodes, cnt)
