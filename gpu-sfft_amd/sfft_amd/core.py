@@ -322,6 +322,7 @@ class Batch:
                                      None, C.byref(h)), "sfft_batch_create")
         self._h = h
         self._keep = {}
+        self._stage = None     # pinned node staging of all signals (grows when needed)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -369,25 +370,39 @@ class Batch:
         _check(lib.sfft_batch_execute_async(self._h, ptrs, dr), "sfft_batch_execute_async")
         self._keep["draws"] = keep
 
-    def execute(self, xs, draws):
-        """Runs the batch; returns a list of (nodes numpy structured array, I_F) per signal."""
+    def execute(self, xs, draws, copy=True):
+        """Runs the batch; returns a list of (nodes numpy structured array, I_F) per signal.
+        The nodes of all signals come back in one round trip into a pinned staging buffer;
+        with copy=False they are views of it (valid until the next execute)."""
         torch = _torch()
         ptrs, dr, keep = self._args(xs, draws)
         counts = np.zeros(self.count, dtype=np.int32)
         torch.cuda.synchronize()
         _check(lib.sfft_batch_execute(self._h, ptrs, dr, None, _np_ptr(counts)),
                "sfft_batch_execute")
-        out = []
-        for i in range(self.count):
-            cnt = int(counts[i])
-            nodes = np.empty(cnt, dtype=NODE_DTYPE)
+        kept = [int(c) for c in counts]   # within capacity (sfft_batch_execute checked)
+        total = sum(kept)
+        if self._stage is None or self._stage.numel() < max(total, 1) * NODE_DTYPE.itemsize:
+            self._stage = torch.empty(max(total, 1) * NODE_DTYPE.itemsize, dtype=torch.uint8,
+                                      pin_memory=True)
+        stage = self._stage.numpy().view(NODE_DTYPE)
+        hip, base, off = _hip(), self._stage.data_ptr(), 0
+        for i, cnt in enumerate(kept):
             if cnt:
                 src = lib.sfft_plan_debug_buffer(lib.sfft_batch_plan(self._h, i), 5)
-                err = _hip().hipMemcpy(_np_ptr(nodes), C.c_void_p(src), C.c_size_t(nodes.nbytes),
-                                       2)
+                err = hip.hipMemcpyAsync(C.c_void_p(base + off * NODE_DTYPE.itemsize),
+                                         C.c_void_p(src), C.c_size_t(cnt * NODE_DTYPE.itemsize),
+                                         2, None)
                 if err != 0:
-                    raise RuntimeError(f"hipMemcpy failed ({err})")
-            out.append((nodes, cnt))
+                    raise RuntimeError(f"hipMemcpyAsync failed ({err})")
+            off += cnt
+        if hip.hipStreamSynchronize(None) != 0:
+            raise RuntimeError("hipStreamSynchronize failed")
+        out, off = [], 0
+        for i, cnt in enumerate(kept):
+            nodes = stage[off:off + cnt]
+            out.append((nodes.copy() if copy else nodes, int(counts[i])))
+            off += cnt
         return out
 
     def tune(self, xs, draws, reps=3):
@@ -410,6 +425,11 @@ def _hip():
         _hip_lib = C.CDLL("libamdhip64.so")
         _hip_lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _hip_lib.hipMemcpy.restype = C.c_int
+        _hip_lib.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
+                                            C.c_void_p]
+        _hip_lib.hipMemcpyAsync.restype = C.c_int
+        _hip_lib.hipStreamSynchronize.argtypes = [C.c_void_p]
+        _hip_lib.hipStreamSynchronize.restype = C.c_int
     return _hip_lib
 
 
