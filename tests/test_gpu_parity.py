@@ -366,6 +366,51 @@ def test_plan_graph_replays_match_eager(S):
             dx.mul_(0.75).add_(torch.roll(dx, 1) * 0.5)
 
 
+def test_plan_graph_output_buffers_change(S):
+    """Graph mode through the C entry point with the output buffer changing between calls:
+    pinned (the binding's buffer, written by the estimation kernel), a plain numpy array
+    (copied into), a second pinned buffer, then the first again.  Each change of output
+    target recaptures the graph; every call equals the eager transform."""
+    import ctypes as C
+    import torch
+    prm = params(SMALL_CONFIGS[1])
+    n = prm["n"]
+    x, _, _ = sparse_signal(n, prm["k"], 8)
+    ft, ff = oracle_filter(prm)
+    win = O.filter_window_slice(ff, n, prm["B"])
+    eager = S.Plan(prm, "fp64")
+    eager.set_filter(_dev(ft), win)
+    graph = S.Plan(prm, "fp64")
+    graph.set_filter(_dev(ft), win)
+    graph.set_graph(True)
+    dx = _dev(x)
+    cap = graph.capacity
+    pinned2 = torch.empty(cap * S.NODE_DTYPE.itemsize, dtype=torch.uint8, pin_memory=True)
+    targets = ["own", "pageable", "pinned2", "own", "pageable"]
+    for rep, tgt in enumerate(targets):
+        a, ai, offs = draws(n, prm["loops"], prm["RLFC_loops"], prm["W_RLFC"], 60 + rep)
+        want, wc = eager.execute(dx, a, ai, offs)
+        if tgt == "own":
+            got, gc = graph.execute(dx, a, ai, offs)
+        else:
+            if tgt == "pageable":
+                buf = np.zeros(cap, dtype=S.NODE_DTYPE)
+                ptr = buf.ctypes.data
+            else:
+                buf = pinned2.numpy().view(S.NODE_DTYPE)
+                ptr = pinned2.data_ptr()
+            dr = graph._draws(a, ai, offs)
+            c = C.c_int(-1)
+            st = S.lib.sfft_execute(graph._h, C.c_void_p(dx.data_ptr()), C.byref(dr),
+                                    C.c_void_p(ptr), C.byref(c))
+            assert st == 0
+            got, gc = buf[:c.value], c.value
+        assert gc == wc, (rep, tgt)
+        assert np.array_equal(got["key"], want["key"]), (rep, tgt)
+        assert np.array_equal(bits(got["re"]), bits(want["re"])), (rep, tgt)
+        assert np.array_equal(bits(got["im"]), bits(want["im"])), (rep, tgt)
+
+
 @pytest.mark.parametrize("cfg", [SMALL_CONFIGS[1], SMALL_CONFIGS[3], N20_K1000])
 def test_comb_keys_bitexact(S, cfg):
     """|Y|^2 of every comb bin (the comb FFT's key epilogue) equals the oracle's samples of
