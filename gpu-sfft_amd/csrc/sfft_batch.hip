@@ -356,6 +356,7 @@ extern "C" sfft_status sfft_batch_tune(sfft_batch* b, const void* const* d_x,
             f.key_batches = P->p.RLFC_loops;
             f.batch = P->p.RLFC_loops;
             f.store = kStoreKeys;
+            f.keys_tiled = 1;
             f.load = kLoadComb;
             f.comb_x = d_x[i];
             f.comb_offs = P->d_offs;

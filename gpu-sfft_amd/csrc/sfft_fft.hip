@@ -120,6 +120,7 @@ struct PassArgs {
   const int* comb_offs;
   int comb_sigma;
   int nbatch;
+  int keys_tiled;   // store keys at tile * TE + e (the tile's own contiguous range)
 };
 
 template <typename T2>
@@ -320,7 +321,7 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt)) void fft_pass_kernel(PassAr
       }
       const T2 val = lds[c * RP + LPAD(kk)];
       if (STORE != kStoreKeys) out[g] = val;
-      if (wkeys) keys[g] = sample_key(val);
+      if (wkeys) keys[A.keys_tiled ? (long long)tile * TE + e : g] = sample_key(val);
     }
   }
 }
@@ -344,6 +345,30 @@ static hipError_t dispatch_pass(const PassArgs& A, int load, int store, int tile
   if (store == kStoreCplx) return launch_pass<Real, kLoadPlain, kStoreCplx>(A, tiles, batch, st);
   if (store == kStoreKeys) return launch_pass<Real, kLoadPlain, kStoreKeys>(A, tiles, batch, st);
   return launch_pass<Real, kLoadPlain, kStoreBoth>(A, tiles, batch, st);
+}
+
+// log2 of the columns per tile of pass p (r bits, after log2Lprev bits): as many as fit the
+// tile, up to qmax; it does not change the arithmetic
+static int pass_log2q(const FftTables& T, const FftLaunch& L, int p, int r, int log2Lprev) {
+  const int qmax = (L.qmax > 0 && L.qmax <= FFT_QMAX) ? L.qmax : FFT_QMAX;
+  const int load = (p == 0) ? L.load : kLoadPlain;
+  const int cap = tile_cap(T.is_fp32, load);
+  int log2Q = 0;
+  while ((1 << (log2Q + 1)) <= qmax && ((1 << (log2Q + 1)) << r) <= cap) ++log2Q;
+  if (p > 0 && log2Q > log2Lprev) log2Q = log2Lprev;
+  return log2Q;
+}
+
+KeyMap fft_keys_map(const FftLaunch& L) {
+  KeyMap m;
+  const FftTables& T = *L.tab;
+  m.log2n = T.log2n;
+  if (!L.keys_tiled || T.npass < 2) return m;
+  const int last = T.npass - 1;
+  const int r = T.logs[last];
+  m.mid = r;
+  m.lo = pass_log2q(T, L, last, r, T.log2n - r);
+  return m;
 }
 
 hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
@@ -376,19 +401,14 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.log2N = log2N;
     A.r = r;
     A.log2Lprev = log2Lprev;
-    // columns per tile: as many as fit the tile, up to qmax (does not change the arithmetic)
-    const int qmax = (L.qmax > 0 && L.qmax <= FFT_QMAX) ? L.qmax : FFT_QMAX;
     const int load = (p == 0) ? L.load : kLoadPlain;
-    const int cap = tile_cap(T.is_fp32, load);
-    int log2Q = 0;
-    while ((1 << (log2Q + 1)) <= qmax && ((1 << (log2Q + 1)) << r) <= cap) ++log2Q;
+    const int log2Q = pass_log2q(T, L, p, r, log2Lprev);
     int tiles;
     if (p == 0) {
       const int lnq = log2N - log2Lcur;
       const int lq = log2Q < lnq ? log2Q : lnq;
       tiles = 1 << (lnq - lq);
     } else {
-      if (log2Q > log2Lprev) log2Q = log2Lprev;
       tiles = (int)((1LL << (log2N - log2Lcur)) << (log2Lprev - log2Q));
     }
     A.log2Q = log2Q;
@@ -402,6 +422,7 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.comb_offs = L.comb_offs;
     A.comb_sigma = L.comb_sigma;
     A.nbatch = L.batch;
+    A.keys_tiled = (last && p > 0 && L.keys_tiled) ? 1 : 0;
     const int store = last ? L.store : kStoreCplx;
     hipError_t e = T.is_fp32 ? dispatch_pass<float>(A, load, store, tiles, L.batch, st)
                              : dispatch_pass<double>(A, load, store, tiles, L.batch, st);

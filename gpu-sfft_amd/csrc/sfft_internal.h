@@ -63,8 +63,25 @@ struct FftLaunch {
   const int* comb_offs = nullptr;
   int comb_sigma = 1;
   int qmax = 0;                  // most columns per tile (power of two; 0 = the default)
+  int keys_tiled = 0;            // keys in the last pass's tile order (see KeyMap)
 };
 hipError_t fft_run(const FftLaunch& L, hipStream_t st);
+
+// Where the keys of an FFT run with keys_tiled land: the last pass of a multi-pass FFT stores
+// its tile's keys contiguously (whole-line stores), so storage position s holds the key of
+// natural index  (s mod 2^lo) | ((s >> lo) mod 2^mid) << (log2n - mid) | (s >> (lo + mid)) << lo
+// (lo: the last pass's columns per tile, mid: its radix bits).  mid == 0: natural order.
+struct KeyMap {
+  int lo = 0, mid = 0, log2n = 0;
+};
+__host__ __device__ inline long long key_natural(const KeyMap& m, long long s) {
+  if (m.mid == 0) return s;
+  const long long c = s & ((1LL << m.lo) - 1);
+  const long long kk = (s >> m.lo) & ((1LL << m.mid) - 1);
+  const long long t = s >> (m.lo + m.mid);
+  return c | (kk << (m.log2n - m.mid)) | (t << m.lo);
+}
+KeyMap fft_keys_map(const FftLaunch& L);
 
 // ---- permutation + filtering -------------------------------------------------------------
 // bins[l*B + r] = canonical fold of loop l (ai_dev[l]) for r < B, l < loops.
@@ -118,10 +135,12 @@ size_t select_work_bytes(int segs, int N);
 void select_work_bind(SelectWork* w, void* base, int segs, int N);
 // out[s*K + j]: the K largest keys of segment s (ties by lower index); ascending indices when
 // `ordered`, otherwise the same set in arbitrary order.  bits (optional, ordered mode): the
-// selection as a bitmap, ceil(N/32) words per segment, every word written
+// selection as a bitmap, ceil(N/32) words per segment, every word written.  map (optional,
+// unordered mode only): the keys are in that storage order; indices (ties, output) are the
+// natural ones
 hipError_t select_topk(int is_fp32, const void* keys, long long seg_stride, int N, int K,
                        int segs, int* out, int ordered, const SelectWork& w, hipStream_t st,
-                       unsigned* bits = nullptr);
+                       unsigned* bits = nullptr, const KeyMap* map = nullptr);
 
 // ---- reverse hash (bitmap of approved residues, deterministic hit list) -----------------
 struct HashArgs {

@@ -400,11 +400,11 @@ HashArgs hash_args(const sfft_plan* P, int* d_count) {
 // mode are recorded by the front and back groups when `marks` is set.
 // front: bitmap clear, streaming pass (option), comb FFT (run_comb), then the comb
 // cutoff and the approved bitmaps (run_cut)
+static FftLaunch comb_fft(const sfft_plan* P, const void* d_x);
 static sfft_status run_comb(sfft_plan* P, const void* d_x, const HashArgs& h, hipStream_t st,
                             bool marks) {
   const sfft_params& p = P->p;
   const int L = p.loops, Rl = p.RLFC_loops;
-  const long long W = p.W_RLFC;
   auto mark = [&](int i) -> hipError_t {
     return (marks && P->profile) ? hipEventRecord(P->ev[i], st) : hipSuccess;
   };
@@ -416,8 +416,17 @@ static sfft_status run_comb(sfft_plan* P, const void* d_x, const HashArgs& h, hi
     HIPCK(stream_scatter_launch(P->fp32, d_x, P->taps, P->comb_y, P->d_a, P->d_offs, p.n, p.w,
                                 L, Rl, p.W_RLFC, st));
   HIPCK(mark(2));
-  // RLFC: comb samples (gathered in pass 0, or from the streaming pass) + W-point FFT, keys
-  // of |Y|^2 out of the last pass
+  HIPCK(fft_run(comb_fft(P, d_x), st));
+  return SFFT_OK;
+}
+
+// RLFC: comb samples (gathered in pass 0, or from the streaming pass) + W-point FFT, keys of
+// |Y|^2 out of the last pass in its tile order (whole-line stores; the comb cutoff maps the
+// storage positions back to the bins)
+static FftLaunch comb_fft(const sfft_plan* P, const void* d_x) {
+  const sfft_params& p = P->p;
+  const int Rl = p.RLFC_loops;
+  const long long W = p.W_RLFC;
   FftLaunch c;
   c.tab = &P->tabW;
   c.out = P->comb0;
@@ -429,6 +438,7 @@ static sfft_status run_comb(sfft_plan* P, const void* d_x, const HashArgs& h, hi
   c.batch = Rl;
   c.store = kStoreKeys;
   c.qmax = P->qmax_W;
+  c.keys_tiled = 1;
   if (P->stream) {
     c.in = P->comb_y;
     c.in_bstride = W;
@@ -439,16 +449,16 @@ static sfft_status run_comb(sfft_plan* P, const void* d_x, const HashArgs& h, hi
     c.comb_offs = P->d_offs;
     c.comb_sigma = (int)(p.n / p.W_RLFC);
   }
-  HIPCK(fft_run(c, st));
-  return SFFT_OK;
+  return c;
 }
 
 // comb cutoff: the comb lists only feed the order-free bitmap build (unordered output)
 static sfft_status run_cut(sfft_plan* P, const HashArgs& h, hipStream_t st, bool marks) {
   const sfft_params& p = P->p;
   const long long W = p.W_RLFC;
+  const KeyMap km = fft_keys_map(comb_fft(P, nullptr));
   HIPCK(select_topk(P->fp32, P->comb_keys, W, (int)W, p.B_thresh, p.RLFC_loops, P->J2, 0,
-                    P->selW, st));
+                    P->selW, st, nullptr, &km));
   HIPCK(hash_bitmap_launch(h, st));
   if (marks && P->profile) HIPCK(hipEventRecord(P->ev[3], st));
   return SFFT_OK;
@@ -788,6 +798,14 @@ extern "C" sfft_status sfft_execute(sfft_plan* P, const void* d_x, const sfft_dr
     HIPCK(hipStreamSynchronize(P->st));
   }
   return cnt > P->cap ? SFFT_ERR_HIT_OVERFLOW : SFFT_OK;
+}
+
+extern "C" sfft_status sfft_plan_comb_key_order(const sfft_plan* P, int* lo, int* mid) {
+  if (!P || !lo || !mid) return SFFT_ERR_INVALID;
+  const KeyMap km = fft_keys_map(comb_fft(P, nullptr));
+  *lo = km.lo;
+  *mid = km.mid;
+  return SFFT_OK;
 }
 
 extern "C" const void* sfft_plan_debug_buffer(const sfft_plan* P, int which) {

@@ -232,7 +232,8 @@ __global__ __launch_bounds__(PICK_NT) void sel_pick_kernel(int Ksel,
 // mode) ---------------------------------------------------------------------------------------
 // Block b writes its candidates to [b*chunk, b*chunk + count) of the segment's candidate
 // arrays, its emitted indices to the same range of the emitted array, and the two counts to
-// ccnt/ecnt[seg][b]: no global atomics (the refine pass compacts).
+// ccnt/ecnt[seg][b]: no global atomics (the refine pass compacts).  Keys stored in a KeyMap
+// order are recorded with their natural indices (the tie rule and the output use those).
 template <typename K>
 __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__ keys,
                                                            long long seg_stride, int N, int chunk,
@@ -242,7 +243,8 @@ __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__
                                                            int* __restrict__ cidx,
                                                            int* __restrict__ eidx,
                                                            int* __restrict__ ccnt,
-                                                           int* __restrict__ ecnt) {
+                                                           int* __restrict__ ecnt,
+                                                           KeyMap km) {
   __shared__ int s_nc, s_ne;
   const int tid = threadIdx.x, seg = blockIdx.y;
   if (tid == 0) {
@@ -266,11 +268,11 @@ __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__
     const int pc = wave_append(&s_nc, isc);
     if (isc) {
       ck[pc] = k;
-      ci[pc] = i;
+      ci[pc] = (int)key_natural(km, i);
     }
     const bool ise = ok && d > d0;
     const int pe = wave_append(&s_ne, ise);
-    if (emit && ise) ce[pe] = i;
+    if (emit && ise) ce[pe] = (int)key_natural(km, i);
   }
   __syncthreads();
   if (tid == 0) {
@@ -554,11 +556,17 @@ __global__ __launch_bounds__(SEL_NT) void sel_write_kernel(const K* __restrict__
 template <typename K>
 static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ksel, int segs,
                               int* out, int ordered, const SelectWork& w, hipStream_t st,
-                              unsigned* bits) {
+                              unsigned* bits, const KeyMap* map) {
   const int chunk = sel_chunk(N);
   const int nchunks = (N + chunk - 1) / chunk;
   if (nchunks > SEL_MAX_CHUNKS || nchunks > w.max_chunks || segs > w.max_segs)
     return hipErrorInvalidValue;
+  KeyMap km;
+  if (map && map->mid != 0) {
+    // the ordered compaction walks storage order: natural order only
+    if (ordered || (1LL << map->log2n) != (long long)N) return hipErrorInvalidValue;
+    km = *map;
+  }
   dim3 grid(nchunks, segs);
   SelState* S = (SelState*)w.state;
   // work layout: column sums, then the per-block histogram rows
@@ -571,7 +579,7 @@ static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ks
                      (const unsigned*)w.hist, S);
   hipLaunchKernelGGL((sel_split_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
                      chunk, nchunks, ordered ? 0 : 1, (const SelState*)S, (K*)w.blk_key,
-                     w.blk_idx, w.emit_idx, w.ccnt, w.ecnt);
+                     w.blk_idx, w.emit_idx, w.ccnt, w.ecnt, km);
   if (ordered) {
     // the refine reads the candidates from the split's per-block regions
     hipLaunchKernelGGL((sel_refine_kernel<K>), dim3(segs), dim3(1024), 0, st, N, Ksel, 0, S,
@@ -595,12 +603,12 @@ static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ks
 
 hipError_t select_topk(int is_fp32, const void* keys, long long seg_stride, int N, int K,
                        int segs, int* out, int ordered, const SelectWork& w, hipStream_t st,
-                       unsigned* bits) {
+                       unsigned* bits, const KeyMap* map) {
   if (is_fp32)
     return select_impl<unsigned int>((const unsigned int*)keys, seg_stride, N, K, segs, out,
-                                     ordered, w, st, bits);
+                                     ordered, w, st, bits, map);
   return select_impl<unsigned long long>((const unsigned long long*)keys, seg_stride, N, K, segs,
-                                         out, ordered, w, st, bits);
+                                         out, ordered, w, st, bits, map);
 }
 
 }  // namespace sfft

@@ -270,6 +270,14 @@ class Plan:
         torch.cuda.synchronize()
         return gx
 
+    def comb_key_order(self):
+        """(lo, mid) of the comb keys' storage order (debug buffer 8): position s holds bin
+        (s mod 2^lo) | ((s >> lo) mod 2^mid) << (log2 W - mid) | (s >> (lo + mid)) << lo."""
+        lo, mid = C.c_int(), C.c_int()
+        _check(lib.sfft_plan_comb_key_order(self._h, C.byref(lo), C.byref(mid)),
+               "sfft_plan_comb_key_order")
+        return lo.value, mid.value
+
     def debug_array(self, which, count, dtype):
         """Copies `count` items of a plan-owned device buffer to a numpy array
         (0 bins_f, 1 J, 2 J2 lists, 3 hits, 4 bins_t, 5 nodes, 6 count, 7 sel, 8 comb keys)."""

@@ -217,8 +217,13 @@ sfft_status sfft_plan_interval_times(sfft_plan *plan, double *seconds7);
  * B_thresh int), 3 hits (int, count from the last execute), 4 bins_t (loops x B complex),
  * 5 nodes (Node[hit_capacity]), 6 count (int), 7 sel (2 unsigned char per hit: the loops
  * whose values the real and the imaginary lower medians returned), 8 comb keys (RLFC_loops x
- * W_RLFC bit patterns of |Y|^2: uint64 for SFFT_FP64, uint32 for SFFT_FP32). */
+ * W_RLFC bit patterns of |Y|^2: uint64 for SFFT_FP64, uint32 for SFFT_FP32), in the storage
+ * order sfft_plan_comb_key_order reports. */
 const void *sfft_plan_debug_buffer(const sfft_plan *plan, int which);
+/* No counterpart in the reference (a debugging aid for the comb keys above): position s of a
+ * comb loop's keys holds bin (s mod 2^lo) | ((s >> lo) mod 2^mid) << (log2 W_RLFC - mid) |
+ * (s >> (lo + mid)) << lo; mid = 0 means natural order. */
+sfft_status sfft_plan_comb_key_order(const sfft_plan *plan, int *lo, int *mid);
 
 /* ---- the value map and its adjoint (FP64 plans) ----------------------------------------
  * No counterpart in the reference: the gradient of the estimated values (innerLoop.cu:

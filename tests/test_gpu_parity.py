@@ -419,7 +419,16 @@ def test_comb_keys_bitexact(S, cfg):
     n, W, Rl = prm["n"], prm["W_RLFC"], prm["RLFC_loops"]
     x, _, _ = sparse_signal(n, prm["k"], cfg[0] + 17)
     _, _, offs = draws(n, prm["loops"], Rl, W, cfg[0] + 17 + 7)
-    keys = plan.debug_array(8, Rl * W, np.uint64).reshape(Rl, W)
+    stored = plan.debug_array(8, Rl * W, np.uint64).reshape(Rl, W)
+    # the keys are stored in the comb FFT's last-pass tile order: back to bin order
+    lo, mid = plan.comb_key_order()
+    s = np.arange(W, dtype=np.int64)
+    lw = W.bit_length() - 1
+    natural = s if mid == 0 else ((s & ((1 << lo) - 1)) | (((s >> lo) & ((1 << mid) - 1)) << (lw - mid))
+                                  | ((s >> (lo + mid)) << lo))
+    assert np.array_equal(np.sort(natural), s)                 # a permutation of the bins
+    keys = np.empty_like(stored)
+    keys[:, natural] = stored
     sigma = n // W
     for r in range(Rl):
         y = x[int(offs[r])::sigma][:W]
