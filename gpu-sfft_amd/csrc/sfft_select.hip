@@ -28,7 +28,8 @@ struct SelState {
   int out_count;
 };
 
-constexpr int SEL_NT = 512;
+constexpr int SEL_NT = 1024;   // threads of the streaming select kernels (512: +11 us comb
+                               // cutoff, +3 us bucket cutoff at n = 2^27, k = 16000)
 constexpr int SEL_DIG = 12;
 constexpr int SEL_BINS = 1 << SEL_DIG;
 
