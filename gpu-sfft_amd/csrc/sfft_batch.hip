@@ -357,6 +357,7 @@ extern "C" sfft_status sfft_batch_tune(sfft_batch* b, const void* const* d_x,
             f.batch = P->p.RLFC_loops;
             f.store = kStoreKeys;
             f.keys_tiled = 1;
+            f.digits = P->comb_digits;
             f.load = kLoadComb;
             f.comb_x = d_x[i];
             f.comb_offs = P->d_offs;

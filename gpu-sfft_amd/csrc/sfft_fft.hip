@@ -123,6 +123,7 @@ struct PassArgs {
   int comb_sigma;
   int nbatch;
   int keys_tiled;   // store keys at tile * TE + e (the tile's own contiguous range)
+  uint16_t* digits; // optional first digits of the keys (same positions)
 };
 
 template <typename T2>
@@ -323,7 +324,14 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt)) void fft_pass_kernel(PassAr
       }
       const T2 val = lds[c * RP + LPAD(kk)];
       if (STORE != kStoreKeys) out[g] = val;
-      if (wkeys) keys[A.keys_tiled ? (long long)tile * TE + e : g] = sample_key(val);
+      if (wkeys) {
+        const long long kpos = A.keys_tiled ? (long long)tile * TE + e : g;
+        const KT key = sample_key(val);
+        keys[kpos] = key;
+        if (A.digits)
+          A.digits[b * A.keys_bstride + kpos] =
+              (uint16_t)((key >> (sizeof(KT) * 8 - 13)) & (KT)0xFFF);
+      }
     }
   }
 }
@@ -425,6 +433,7 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.comb_sigma = L.comb_sigma;
     A.nbatch = L.batch;
     A.keys_tiled = (last && p > 0 && L.keys_tiled) ? 1 : 0;
+    A.digits = last ? L.digits : nullptr;
     const int store = last ? L.store : kStoreCplx;
     hipError_t e = T.is_fp32 ? dispatch_pass<float>(A, load, store, tiles, L.batch, st)
                              : dispatch_pass<double>(A, load, store, tiles, L.batch, st);

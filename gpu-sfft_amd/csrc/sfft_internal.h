@@ -64,6 +64,8 @@ struct FftLaunch {
   int comb_sigma = 1;
   int qmax = 0;                  // most columns per tile (power of two; 0 = the default)
   int keys_tiled = 0;            // keys in the last pass's tile order (see KeyMap)
+  uint16_t* digits = nullptr;    // optional: each key's top 12 bits below the sign bit (the
+                                 // select's first digit), same positions and stride as keys
 };
 hipError_t fft_run(const FftLaunch& L, hipStream_t st);
 
@@ -138,9 +140,12 @@ void select_work_bind(SelectWork* w, void* base, int segs, int N);
 // selection as a bitmap, ceil(N/32) words per segment, every word written.  map (optional,
 // unordered mode only): the keys are in that storage order; indices (ties, output) are the
 // natural ones
+// digits (optional, unordered mode only): the keys' first digits (as the FFT's `digits`
+// output), read instead of the keys wherever the first digit is all a pass needs
 hipError_t select_topk(int is_fp32, const void* keys, long long seg_stride, int N, int K,
                        int segs, int* out, int ordered, const SelectWork& w, hipStream_t st,
-                       unsigned* bits = nullptr, const KeyMap* map = nullptr);
+                       unsigned* bits = nullptr, const KeyMap* map = nullptr,
+                       const uint16_t* digits = nullptr);
 
 // ---- reverse hash (bitmap of approved residues, deterministic hit list) -----------------
 struct HashArgs {

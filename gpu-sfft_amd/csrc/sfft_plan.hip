@@ -258,6 +258,7 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   items.push_back({&P->comb0, (size_t)Rl * W * e});
   items.push_back({&P->comb1, (size_t)Rl * W * e});
   items.push_back({&P->comb_keys, (size_t)Rl * W * ke});
+  items.push_back({(void**)&P->comb_digits, (size_t)Rl * W * 2});
   items.push_back({&P->bins_t, (size_t)L * B * e});
   items.push_back({&P->bins_scr, (size_t)L * B * e});
   items.push_back({&P->bins_f, (size_t)L * B * e});
@@ -439,6 +440,7 @@ static FftLaunch comb_fft(const sfft_plan* P, const void* d_x) {
   c.store = kStoreKeys;
   c.qmax = P->qmax_W;
   c.keys_tiled = 1;
+  c.digits = P->comb_digits;
   if (P->stream) {
     c.in = P->comb_y;
     c.in_bstride = W;
@@ -458,7 +460,7 @@ static sfft_status run_cut(sfft_plan* P, const HashArgs& h, hipStream_t st, bool
   const long long W = p.W_RLFC;
   const KeyMap km = fft_keys_map(comb_fft(P, nullptr));
   HIPCK(select_topk(P->fp32, P->comb_keys, W, (int)W, p.B_thresh, p.RLFC_loops, P->J2, 0,
-                    P->selW, st, nullptr, &km));
+                    P->selW, st, nullptr, &km, P->comb_digits));
   HIPCK(hash_bitmap_launch(h, st));
   if (marks && P->profile) HIPCK(hipEventRecord(P->ev[3], st));
   return SFFT_OK;

@@ -23,6 +23,7 @@ struct sfft_plan {
   void* comb0 = nullptr;
   void* comb1 = nullptr;
   void* comb_keys = nullptr;
+  uint16_t* comb_digits = nullptr;   // first select digit of each comb key (same order)
   int stream = 0;               // streaming front end (else gathers)
   void* taps = nullptr;         // loops x w tap values (stream)
   void* comb_y = nullptr;       // RLFC_loops x W comb samples (stream)

@@ -146,7 +146,8 @@ template <typename K>
 __global__ __launch_bounds__(SEL_NT) void sel_hist0_kernel(const K* __restrict__ keys,
                                                            long long seg_stride, int N, int chunk,
                                                            int nchunks,
-                                                           unsigned* __restrict__ rows) {
+                                                           unsigned* __restrict__ rows,
+                                                           const uint16_t* __restrict__ digits) {
   __shared__ unsigned h[SEL_BINS];
   const int tid = threadIdx.x, seg = blockIdx.y;
   for (int i = tid; i < SEL_BINS; i += SEL_NT) h[i] = 0u;
@@ -154,17 +155,34 @@ __global__ __launch_bounds__(SEL_NT) void sel_hist0_kernel(const K* __restrict__
   const K* kp = keys + (long long)seg * seg_stride;
   const int begin = blockIdx.x * chunk, end = min(N, begin + chunk);
   constexpr int U = 8;   // independent loads in flight per thread
-  for (int base = begin; base < end; base += SEL_NT * U) {
-    K kk[U];
+  if (digits) {          // the first digits only (2 bytes per key instead of 4 or 8)
+    const uint16_t* dp = digits + (long long)seg * seg_stride;
+    for (int base = begin; base < end; base += SEL_NT * U) {
+      unsigned dd[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = base + u * SEL_NT + tid;
-      kk[u] = i < end ? kp[i] : (K)0;
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * SEL_NT + tid;
+        dd[u] = i < end ? (unsigned)dp[i] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * SEL_NT + tid;
+        wave_hist_add(h, dd[u], i < end);
+      }
     }
+  } else {
+    for (int base = begin; base < end; base += SEL_NT * U) {
+      K kk[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = base + u * SEL_NT + tid;
-      wave_hist_add(h, top_digit<K>(kk[u]), i < end);
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * SEL_NT + tid;
+        kk[u] = i < end ? kp[i] : (K)0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * SEL_NT + tid;
+        wave_hist_add(h, top_digit<K>(kk[u]), i < end);
+      }
     }
   }
   __syncthreads();
@@ -245,7 +263,8 @@ __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__
                                                            int* __restrict__ eidx,
                                                            int* __restrict__ ccnt,
                                                            int* __restrict__ ecnt,
-                                                           KeyMap km) {
+                                                           KeyMap km,
+                                                           const uint16_t* __restrict__ digits) {
   __shared__ int s_nc, s_ne;
   const int tid = threadIdx.x, seg = blockIdx.y;
   if (tid == 0) {
@@ -260,12 +279,14 @@ __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__
   int* ci = cidx + rb;
   int* ce = eidx + rb;
   const int begin = blockIdx.x * chunk, end = min(N, begin + chunk);
+  const uint16_t* dp = digits ? digits + (long long)seg * seg_stride : nullptr;
   for (int base = begin; base < end; base += SEL_NT) {
     const int i = base + tid;
     const bool ok = i < end;
-    const K k = ok ? kp[i] : (K)0;
-    const unsigned d = top_digit<K>(k);
+    // with the first digits at hand, a key is read only when it is a candidate
+    const unsigned d = dp ? (ok ? (unsigned)dp[i] : 0u) : top_digit<K>(ok ? kp[i] : (K)0);
     const bool isc = ok && d == d0;
+    const K k = isc ? kp[i] : (K)0;
     const int pc = wave_append(&s_nc, isc);
     if (isc) {
       ck[pc] = k;
@@ -557,7 +578,7 @@ __global__ __launch_bounds__(SEL_NT) void sel_write_kernel(const K* __restrict__
 template <typename K>
 static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ksel, int segs,
                               int* out, int ordered, const SelectWork& w, hipStream_t st,
-                              unsigned* bits, const KeyMap* map) {
+                              unsigned* bits, const KeyMap* map, const uint16_t* digits) {
   const int chunk = sel_chunk(N);
   const int nchunks = (N + chunk - 1) / chunk;
   if (nchunks > SEL_MAX_CHUNKS || nchunks > w.max_chunks || segs > w.max_segs)
@@ -573,14 +594,14 @@ static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ks
   // work layout: column sums, then the per-block histogram rows
   unsigned* rows = w.hist + (size_t)segs * SEL_BINS;
   hipLaunchKernelGGL((sel_hist0_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
-                     chunk, nchunks, rows);
+                     chunk, nchunks, rows, digits);
   hipLaunchKernelGGL(sel_colsum_kernel, dim3(SEL_BINS / PICK_BINS, segs), dim3(PICK_NT), 0, st,
                      nchunks, (const unsigned*)rows, w.hist);
   hipLaunchKernelGGL(sel_pick_kernel, dim3(segs), dim3(PICK_NT), 0, st, Ksel,
                      (const unsigned*)w.hist, S);
   hipLaunchKernelGGL((sel_split_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
                      chunk, nchunks, ordered ? 0 : 1, (const SelState*)S, (K*)w.blk_key,
-                     w.blk_idx, w.emit_idx, w.ccnt, w.ecnt, km);
+                     w.blk_idx, w.emit_idx, w.ccnt, w.ecnt, km, digits);
   if (ordered) {
     // the refine reads the candidates from the split's per-block regions
     hipLaunchKernelGGL((sel_refine_kernel<K>), dim3(segs), dim3(1024), 0, st, N, Ksel, 0, S,
@@ -604,12 +625,13 @@ static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ks
 
 hipError_t select_topk(int is_fp32, const void* keys, long long seg_stride, int N, int K,
                        int segs, int* out, int ordered, const SelectWork& w, hipStream_t st,
-                       unsigned* bits, const KeyMap* map) {
+                       unsigned* bits, const KeyMap* map, const uint16_t* digits) {
+  if (ordered && digits) return hipErrorInvalidValue;   // the write pass reads the keys
   if (is_fp32)
     return select_impl<unsigned int>((const unsigned int*)keys, seg_stride, N, K, segs, out,
-                                     ordered, w, st, bits, map);
+                                     ordered, w, st, bits, map, digits);
   return select_impl<unsigned long long>((const unsigned long long*)keys, seg_stride, N, K, segs,
-                                         out, ordered, w, st, bits, map);
+                                         out, ordered, w, st, bits, map, digits);
 }
 
 }  // namespace sfft
