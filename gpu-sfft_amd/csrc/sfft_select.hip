@@ -3,10 +3,12 @@
 //
 // Every element gets a unique composite key  c = (key without sign bit) . (N-1-index)  so that
 // "larger c" means "larger |X|^2, or equal and lower index"; the top-K set is {c >= T}.
-//   1. hist0   : histogram of the top 12 bits of c over all N (wave-aggregated LDS atomics);
-//                the last block of a segment picks the digit d0 holding the K-th largest.
-//   2. split   : one pass over N: digit > d0 -> selected (emitted at once in unordered mode),
-//                digit == d0 -> appended to a candidate list (typically ~1% of N).
+//   1. hist0   : histogram of the top 12 bits of c, one row per block (wave-aggregated LDS
+//                atomics), then colsum: the column sums of the rows.
+//   2. split   : every block picks the digit d0 holding the K-th largest from the column
+//                sums, then one pass over N: digit > d0 -> selected (emitted at once in
+//                unordered mode), digit == d0 -> appended to a candidate list (typically ~1%
+//                of N).
 //   3. refine  : one workgroup per segment: MSD radix select on the remaining bits of c over
 //                the candidates only, with early exit once a digit bin fills the remainder;
 //                yields T and (unordered mode) emits the selected candidates.
@@ -216,37 +218,6 @@ __global__ __launch_bounds__(PICK_NT) void sel_colsum_kernel(int nchunks,
   }
 }
 
-// ---- 1c. pick the top digit d0 holding the K-th largest (one block per segment) -----------
-__global__ __launch_bounds__(PICK_NT) void sel_pick_kernel(int Ksel,
-                                                           const unsigned* __restrict__ ghist,
-                                                           SelState* __restrict__ st) {
-  __shared__ int sh[16];
-  const int tid = threadIdx.x, seg = blockIdx.x;
-  const unsigned* gh = ghist + (size_t)seg * SEL_BINS;
-  // bins from the top: thread t owns ranks [16t, 16t+16)
-  constexpr int PER = SEL_BINS / PICK_NT;
-  unsigned v[PER];
-  int S = 0;
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    v[j] = gh[SEL_BINS - 1 - (PER * tid + j)];
-    S += (int)v[j];
-  }
-  const int E = block_excl_scan_s(S, sh);
-  if (E < Ksel && Ksel <= E + S) {
-    int cum = E;
-#pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      if (cum + (int)v[j] >= Ksel) {
-        st[seg].d0 = SEL_BINS - 1 - (PER * tid + j);
-        st[seg].need0 = Ksel - cum;
-        break;
-      }
-      cum += (int)v[j];
-    }
-  }
-}
-
 // ---- 2. split: collect digit == d0 (candidates), count digit > d0 (and emit them, unordered
 // mode) ---------------------------------------------------------------------------------------
 // Block b writes its candidates to [b*chunk, b*chunk + count) of the segment's candidate
@@ -256,8 +227,9 @@ __global__ __launch_bounds__(PICK_NT) void sel_pick_kernel(int Ksel,
 template <typename K>
 __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__ keys,
                                                            long long seg_stride, int N, int chunk,
-                                                           int nchunks, int emit,
-                                                           const SelState* __restrict__ st,
+                                                           int nchunks, int emit, int Ksel,
+                                                           const unsigned* __restrict__ ghist,
+                                                           SelState* __restrict__ st,
                                                            K* __restrict__ ckey,
                                                            int* __restrict__ cidx,
                                                            int* __restrict__ eidx,
@@ -265,14 +237,47 @@ __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__
                                                            int* __restrict__ ecnt,
                                                            KeyMap km,
                                                            const uint16_t* __restrict__ digits) {
-  __shared__ int s_nc, s_ne;
+  static_assert(SEL_NT * 4 == SEL_BINS, "the pick gives each thread four bins");
+  __shared__ int s_nc, s_ne, s_d0;
+  __shared__ int sh[16];
   const int tid = threadIdx.x, seg = blockIdx.y;
   if (tid == 0) {
     s_nc = 0;
     s_ne = 0;
+    s_d0 = 0;
+  }
+  // the top digit d0 holding the K-th largest key, from the column sums: every block finds
+  // the same one (16 KB of sums, no separate pick launch); block 0 records it and the count
+  // still needed inside it for the refine
+  {
+    const unsigned* gh = ghist + (size_t)seg * SEL_BINS;
+    unsigned v[4];
+    int S = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v[j] = gh[SEL_BINS - 1 - (4 * tid + j)];
+      S += (int)v[j];
+    }
+    const int E = block_excl_scan_s(S, sh);   // (its barriers also publish the defaults)
+    if (E < Ksel && Ksel <= E + S) {
+      int cum = E;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (cum + (int)v[j] >= Ksel) {
+          const int dd = SEL_BINS - 1 - (4 * tid + j);
+          s_d0 = dd;
+          if (blockIdx.x == 0) {
+            st[seg].d0 = dd;
+            st[seg].need0 = Ksel - cum;
+          }
+          break;
+        }
+        cum += (int)v[j];
+      }
+    }
   }
   __syncthreads();
-  const unsigned d0 = (unsigned)st[seg].d0;
+  const unsigned d0 = (unsigned)s_d0;
   const K* kp = keys + (long long)seg * seg_stride;
   const long long rb = (long long)seg * N + (long long)blockIdx.x * chunk;
   K* ck = ckey + rb;
@@ -597,10 +602,10 @@ static hipError_t select_impl(const K* keys, long long seg_stride, int N, int Ks
                      chunk, nchunks, rows, digits);
   hipLaunchKernelGGL(sel_colsum_kernel, dim3(SEL_BINS / PICK_BINS, segs), dim3(PICK_NT), 0, st,
                      nchunks, (const unsigned*)rows, w.hist);
-  hipLaunchKernelGGL(sel_pick_kernel, dim3(segs), dim3(PICK_NT), 0, st, Ksel,
-                     (const unsigned*)w.hist, S);
+  // (the split blocks pick d0 from the column sums themselves)
   hipLaunchKernelGGL((sel_split_kernel<K>), grid, dim3(SEL_NT), 0, st, keys, seg_stride, N,
-                     chunk, nchunks, ordered ? 0 : 1, (const SelState*)S, (K*)w.blk_key,
+                     chunk, nchunks, ordered ? 0 : 1, Ksel, (const unsigned*)w.hist, S,
+                     (K*)w.blk_key,
                      w.blk_idx, w.emit_idx, w.ccnt, w.ecnt, km, digits);
   if (ordered) {
     // the refine reads the candidates from the split's per-block regions
