@@ -632,6 +632,7 @@ hipError_t select_topk(int is_fp32, const void* keys, long long seg_stride, int 
                        int segs, int* out, int ordered, const SelectWork& w, hipStream_t st,
                        unsigned* bits, const KeyMap* map, const uint16_t* digits) {
   if (ordered && digits) return hipErrorInvalidValue;   // the write pass reads the keys
+  if (N < 2 || K < 1 || K > N || segs < 1) return hipErrorInvalidValue;
   if (is_fp32)
     return select_impl<unsigned int>((const unsigned int*)keys, seg_stride, N, K, segs, out,
                                      ordered, w, st, bits, map, digits);
