@@ -199,7 +199,8 @@ def test_transform_degenerate_signals_ties(S, kind):
 
 
 @pytest.mark.parametrize("N,K", [(1 << 10, 100), (1 << 14, 4000), (1 << 17, 32000),
-                                 (1000, 100), (5000, 1234), (70001, 30000)])
+                                 (1000, 100), (5000, 1234), (70001, 30000),
+                                 (1 << 10, 1), (1 << 10, (1 << 10) - 1), (3, 2)])
 def test_samples_indices_with_ties(S, N, K):
     """Reference entry point samples_indices (innerLoop.cu:484-539) on keys with ties, at
     power-of-two and other sizes (the index field of the composite key needs ceil(log2 N)
