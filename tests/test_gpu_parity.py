@@ -316,6 +316,17 @@ def test_reference_outer_loop_entry_point(S):
 @pytest.mark.parametrize("cfg", [SMALL_CONFIGS[1], SMALL_CONFIGS[3], N20_K1000])
 @pytest.mark.parametrize("front_end", ["gather", "stream"])
 def test_transform_fp32_bitexact(S, cfg, front_end):
+    _fp32_transform_check(S, cfg, front_end)
+
+
+@pytest.mark.parametrize("loops", LOOP_VARIANTS, ids=lambda v: "loc%d_thr%d_est%d" % v)
+def test_transform_fp32_loop_counts_bitexact(S, loops):
+    """the FP32 path (its own estimation kernels: float bucket rows) at the loop counts and
+    thresholds of test_transform_loop_counts_bitexact"""
+    _fp32_transform_check(S, (14, 20, 1.0) + tuple(loops) + (4, 2), "gather")
+
+
+def _fp32_transform_check(S, cfg, front_end):
     prm = params(cfg)
     n, B, L, Ll, Rl, K = (prm[k] for k in ("n", "B", "loops", "loops_loc", "RLFC_loops",
                                             "B_thresh"))
