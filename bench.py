@@ -186,11 +186,12 @@ def main():
         # the same kernel priced with a 64-byte granule (half a line per gather)
         b64 = L * w * 64 + w * e + L * B * e
         roofline["achieved_granule64"] = round(b64 / (k_ms * 1e-3) / 1e9, 1)
-    # whole-transform algorithmic rate (SURVEY formula: gathers + taps + nodes)
+    # whole-transform algorithmic rate: one 128-B line per gathered tap and comb sample (at most
+    # the whole signal), the taps, and the nodes (24-byte FP64 nodes in both precisions)
     sig = n // prm["W_RLFC"]
     g_c = GATHER_GRANULE if sig * e >= GATHER_GRANULE else sig * e
     x_bytes = min(prm["RLFC_loops"] * prm["W_RLFC"] * g_c + L * w * GATHER_GRANULE, n * e)
-    alg = x_bytes + w * e + cnt * (24 if not fp32 else 16)
+    alg = x_bytes + w * e + cnt * 24
     transform = {"alg_bytes": alg, "GB/s": round(alg / (ms_per_step * 1e-3) / 1e9, 1),
                  "frac": round(alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
