@@ -52,8 +52,8 @@ __global__ __launch_bounds__(256) void value_pick_kernel(
   const double2 br = bins_f[(long long)lr * B + hr];
   const double2 bi = bins_f[(long long)li * B + hi];
   double2 v;
-  v.x = fma(br.x, Fr.x, br.y * Fr.y) / fma(Fr.x, Fr.x, Fr.y * Fr.y);
-  v.y = fma(bi.y, Fi.x, -(bi.x * Fi.y)) / fma(Fi.x, Fi.x, Fi.y * Fi.y);
+  v.x = (br.x * Fr.x + br.y * Fr.y) / (Fr.x * Fr.x + Fr.y * Fr.y);
+  v.y = (bi.y * Fi.x - bi.x * Fi.y) / (Fi.x * Fi.x + Fi.y * Fi.y);
   values[h] = v;
 }
 
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void adj_sum_kernel(
     long long hb;
     double2 F;
     est_term((uint32_t)hits[best >> 1], l, n, ai_dev, B, win, &hb, &F);
-    const double d = fma(F.x, F.x, F.y * F.y);
+    const double d = F.x * F.x + F.y * F.y;
     const double2 gv = g[best >> 1];
     if ((best & 1) == 0) {
       const double q = gv.x / d;

@@ -251,7 +251,7 @@ static int cutoff_device(int fp32, const void* d_keys, int N, int K, int* d_out,
 __global__ void keys_c128_kernel(const double2* __restrict__ X, unsigned long long* __restrict__ k,
                                  int N) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < N) k[i] = (unsigned long long)__double_as_longlong(fma(X[i].x, X[i].x, X[i].y * X[i].y));
+  if (i < N) k[i] = (unsigned long long)__double_as_longlong(X[i].x * X[i].x + X[i].y * X[i].y);
 }
 
 extern "C" int samples_indices(complex_t* d_samp, int B, int samp_size, int* d_output,

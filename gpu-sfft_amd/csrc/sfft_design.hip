@@ -460,7 +460,7 @@ extern "C" sfft_status sfft_metric(const Node* nodes, int I_F, int k, const int*
                                    complex_t* d_X, int n, int* missed_est, int* missed_res,
                                    double* err_total, double* err_support, double* max_abs,
                                    void* stream) {
-  if (!nodes || I_F < 0 || k < 1 || !locs || !d_X) return SFFT_ERR_INVALID;
+  if ((!nodes && I_F > 0) || I_F < 0 || k < 1 || !locs || !d_X) return SFFT_ERR_INVALID;
   for (int i = 0; i < k; ++i)
     if (locs[i] < 0 || locs[i] >= n) return SFFT_ERR_INVALID;
   for (int i = 0; i < I_F; ++i)

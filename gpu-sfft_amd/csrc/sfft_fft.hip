@@ -31,11 +31,13 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 w) {
 __device__ __forceinline__ float2 cmul(float2 a, float2 w) {
   return make_float2(fmaf(a.x, w.x, -(a.y * w.y)), fmaf(a.x, w.y, a.y * w.x));
 }
+// |X|^2 as the reference's pow(re,2) + pow(im,2) (innerLoop.cu:715): each square rounded,
+// then added
 __device__ __forceinline__ unsigned long long sample_key(double2 v) {
-  return (unsigned long long)__double_as_longlong(fma(v.x, v.x, v.y * v.y));
+  return (unsigned long long)__double_as_longlong(v.x * v.x + v.y * v.y);
 }
 __device__ __forceinline__ unsigned int sample_key(float2 v) {
-  return __float_as_uint(fmaf(v.x, v.x, v.y * v.y));
+  return __float_as_uint(v.x * v.x + v.y * v.y);
 }
 
 template <typename T2>
@@ -448,14 +450,16 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
 // ============================================================================================
 // permutation + filtering (bucket-owner fold)
 // ============================================================================================
+// The term as the reference writes it (innerLoop.cu:740-741, 770-771): both products
+// rounded, combined, then added to the bucket sum.
 struct Acc64 {
   double2 s;
   __device__ __forceinline__ void init() { s = make_double2(0.0, 0.0); }
   __device__ __forceinline__ void add(double2 x, double2 f) {
-    s.x = fma(x.x, f.x, s.x);
-    s.x = fma(-x.y, f.y, s.x);
-    s.y = fma(x.x, f.y, s.y);
-    s.y = fma(x.y, f.x, s.y);
+    const double tx = x.x * f.x - x.y * f.y;
+    const double ty = x.x * f.y + x.y * f.x;
+    s.x = s.x + tx;
+    s.y = s.y + ty;
   }
   __device__ __forceinline__ double2 get() const { return s; }
 };
@@ -470,8 +474,8 @@ struct Acc32 {  // Kahan-compensated FP32 fold
     sum = t;
   }
   __device__ __forceinline__ void add(float2 x, float2 f) {
-    const float tx = fmaf(x.x, f.x, -(x.y * f.y));
-    const float ty = fmaf(x.x, f.y, x.y * f.x);
+    const float tx = x.x * f.x - x.y * f.y;
+    const float ty = x.x * f.y + x.y * f.x;
     kahan(sx, cx, tx);
     kahan(sy, cy, ty);
   }

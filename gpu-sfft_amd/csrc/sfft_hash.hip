@@ -404,9 +404,10 @@ __global__ __launch_bounds__(EST_NT) void estimate_kernel(
     double vr[MAXL], vi[MAXL];
 #pragma unroll
     for (int l = 0; l < MAXL; ++l) {
-      const double abs2 = fma(F[l].x, F[l].x, F[l].y * F[l].y);
-      const double mr = fma(bb[l].x, F[l].x, bb[l].y * F[l].y);
-      const double mi = fma(bb[l].y, F[l].x, -(bb[l].x * F[l].y));
+      // innerLoop.cu:848-852: products rounded, then combined
+      const double abs2 = F[l].x * F[l].x + F[l].y * F[l].y;
+      const double mr = bb[l].x * F[l].x + bb[l].y * F[l].y;
+      const double mi = bb[l].y * F[l].x - bb[l].x * F[l].y;
       vr[l] = mr / abs2;
       vi[l] = mi / abs2;
     }

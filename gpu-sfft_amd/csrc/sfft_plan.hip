@@ -346,6 +346,14 @@ extern "C" size_t sfft_plan_workspace_bytes(const sfft_plan* P) { return P ? P->
 extern "C" sfft_status sfft_plan_set_filter(sfft_plan* P, const void* d_filter_time,
                                             const complex_t* h_filter_win) {
   if (!P || !d_filter_time || !h_filter_win) return SFFT_ERR_INVALID;
+  // A transform still running on the plan's streams reads the old taps and window: let it
+  // finish before the window is rewritten (and before the caller may free the old taps).  A
+  // captured graph holds the old taps pointer as a kernel argument: drop it, so the next
+  // graph-mode execute captures the new one.
+  if (P->st) HIPCK(hipStreamSynchronize(P->st));
+  else HIPCK(hipDeviceSynchronize());
+  if (P->st2) HIPCK(hipStreamSynchronize(P->st2));
+  drop_plan_graph(P);
   if (hipMemcpy(P->win, h_filter_win, (size_t)P->m * 16, hipMemcpyHostToDevice) != hipSuccess)
     return SFFT_ERR_HIP;
   P->d_ft = d_filter_time;

@@ -300,11 +300,13 @@ void orc_make_multiple_t_taps(double lobefrac, double tolerance, int n, int b_f,
 /* ------------------------------------------------------------------------------------------
  * permutation + filtering (innerLoop.cu:194-246, 725-773)
  * ---------------------------------------------------------------------------------------- */
+/* the term as the reference writes it (innerLoop.cu:740-741, 770-771): both products
+ * rounded, combined, then added to the bucket (no fused multiply-add) */
 static inline void cmac64(orc_c64 *acc, orc_c64 x, orc_c64 f) {
-  acc->x = fma(x.x, f.x, acc->x);
-  acc->x = fma(-x.y, f.y, acc->x);
-  acc->y = fma(x.x, f.y, acc->y);
-  acc->y = fma(x.y, f.x, acc->y);
+  const double tx = x.x * f.x - x.y * f.y;
+  const double ty = x.x * f.y + x.y * f.x;
+  acc->x = acc->x + tx;
+  acc->y = acc->y + ty;
 }
 
 void orc_perm_filter64(const orc_c64 *x, int n, const orc_c64 *f, int w, int B, int ai,
@@ -333,7 +335,8 @@ void orc_perm_filter64(const orc_c64 *x, int n, const orc_c64 *f, int w, int B, 
   }
 }
 
-/* FP32: term in FP32 with one fma per component, Kahan-compensated accumulation */
+/* FP32: the same term in FP32 (products rounded, then combined), Kahan-compensated
+ * accumulation */
 static inline void kahan_add(float *sum, float *comp, float term) {
   const float y = term - *comp;
   const float t = *sum + y;
@@ -342,8 +345,8 @@ static inline void kahan_add(float *sum, float *comp, float term) {
 }
 
 static inline void cmac32k(float *sx, float *cx, float *sy, float *cy, orc_c32 x, orc_c32 f) {
-  const float tx = fmaf(x.x, f.x, -(x.y * f.y));
-  const float ty = fmaf(x.x, f.y, x.y * f.x);
+  const float tx = x.x * f.x - x.y * f.y;
+  const float ty = x.x * f.y + x.y * f.x;
   kahan_add(sx, cx, tx);
   kahan_add(sy, cy, ty);
 }
@@ -378,12 +381,14 @@ void orc_perm_filter32(const orc_c32 *x, int n, const orc_c32 *f, int w, int B, 
 /* ------------------------------------------------------------------------------------------
  * samples and cutoff (innerLoop.cu:484-642, 708-718, 875-890)
  * ---------------------------------------------------------------------------------------- */
+/* |X|^2 = re*re + im*im, the reference's pow(re,2) + pow(im,2) (innerLoop.cu:715) with
+ * each square rounded and then added */
 void orc_samples64(const orc_c64 *X, int N, double *s) {
-  for (int i = 0; i < N; ++i) s[i] = fma(X[i].x, X[i].x, X[i].y * X[i].y);
+  for (int i = 0; i < N; ++i) s[i] = X[i].x * X[i].x + X[i].y * X[i].y;
 }
 
 void orc_samples32(const orc_c32 *X, int N, float *s) {
-  for (int i = 0; i < N; ++i) s[i] = fmaf(X[i].x, X[i].x, X[i].y * X[i].y);
+  for (int i = 0; i < N; ++i) s[i] = X[i].x * X[i].x + X[i].y * X[i].y;
 }
 
 /* K-th largest by quickselect on a copy (the reference sorts; the order statistic is the
@@ -570,9 +575,10 @@ static void estimate_values_any(const int *hits, int I_F, const orc_c64 *bins_f,
         b.x = (double)bins32[(size_t)l * B + hashed].x;
         b.y = (double)bins32[(size_t)l * B + hashed].y;
       }
-      const double abs2 = fma(F.x, F.x, F.y * F.y);
-      const double mr = fma(b.x, F.x, b.y * F.y);
-      const double mi = fma(b.y, F.x, -(b.x * F.y));
+      /* innerLoop.cu:848-852: |F|^2, then bins * conj(F), products rounded and combined */
+      const double abs2 = F.x * F.x + F.y * F.y;
+      const double mr = b.x * F.x + b.y * F.y;
+      const double mi = b.y * F.x - b.x * F.y;
       vr[l] = mr / abs2;
       vi[l] = mi / abs2;
     }
@@ -797,9 +803,9 @@ void orc_estimate_select(const int *hits, int I_F, const orc_c64 *bins_f, int lo
       orc_c64 F;
       est_term(hits[h], l, n, ai, B, filter_win, &hashed, &F);
       const orc_c64 b = bins_f[(size_t)l * B + hashed];
-      const double abs2 = fma(F.x, F.x, F.y * F.y);
-      vr[l] = fma(b.x, F.x, b.y * F.y) / abs2;
-      vi[l] = fma(b.y, F.x, -(b.x * F.y)) / abs2;
+      const double abs2 = F.x * F.x + F.y * F.y;
+      vr[l] = (b.x * F.x + b.y * F.y) / abs2;
+      vi[l] = (b.y * F.x - b.x * F.y) / abs2;
     }
     /* the loop a stable insertion sort puts at position med */
     for (int j = 0; j < loops; ++j) {
@@ -841,8 +847,8 @@ void orc_value_apply64(const orc_plan *p, const orc_c64 *x, const orc_c64 *f,
     est_term(hits[h], sel[2 * h + 1], p->n, ai, B, win, &hi, &Fi);
     const orc_c64 br = bins_f[(size_t)sel[2 * h] * B + hr];
     const orc_c64 bi = bins_f[(size_t)sel[2 * h + 1] * B + hi];
-    out[h].x = fma(br.x, Fr.x, br.y * Fr.y) / fma(Fr.x, Fr.x, Fr.y * Fr.y);
-    out[h].y = fma(bi.y, Fi.x, -(bi.x * Fi.y)) / fma(Fi.x, Fi.x, Fi.y * Fi.y);
+    out[h].x = (br.x * Fr.x + br.y * Fr.y) / (Fr.x * Fr.x + Fr.y * Fr.y);
+    out[h].y = (bi.y * Fi.x - bi.x * Fi.y) / (Fi.x * Fi.x + Fi.y * Fi.y);
   }
   free(bins_f);
 }
@@ -859,7 +865,7 @@ void orc_value_adjoint64(const orc_plan *p, const orc_c64 *f, const orc_c64 *win
       long long hb;
       orc_c64 F;
       est_term(hits[h], l, n, ai, B, win, &hb, &F);
-      const double d = fma(F.x, F.x, F.y * F.y);
+      const double d = F.x * F.x + F.y * F.y;
       orc_c64 *ub = &u[(size_t)l * B + hb];
       if (part == 0) {
         const double q = g[h].x / d;

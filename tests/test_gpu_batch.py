@@ -119,3 +119,60 @@ def test_batch_fp64_matches_oracle(S):
         assert np.array_equal(nodes["key"], ref["nodes"]["key"])
         assert np.array_equal(bits(nodes["re"]), bits(ref["nodes"]["re"]))
         assert np.array_equal(bits(nodes["im"]), bits(ref["nodes"]["im"]))
+
+
+def _workload64(S, seed):
+    """The batched workload at its real size: 64 complex64 signals, n = 2^q with q uniform on
+    18..24, k log-uniform on [50, 4000], unit-magnitude random-phase spectra, 40 dB noise,
+    Bcst 1.0, loops 2+5, thresh 2, RLFC 256 x 2; filters from the parallel device design."""
+    import torch
+    prms = S.batch_workload(64, seed)
+    xs, fts, wins, draws = [], [], [], []
+    for i, p in enumerate(prms):
+        n, k = p["n"], p["k"]
+        locs, vals = S.signal_spectrum(n, k, 1000 * seed + i)
+        xs.append(S.synthesize(n, k, locs, vals, snr_db=40.0, noise_seed=i + 1, fp32=True))
+        ft, ff = S.make_filter(p["lobefrac"], p["tolerance"], n, p["b_f"], p["w"],
+                               mode="parallel", freq="device")
+        fts.append(S.to_complex64(ft))
+        wins.append(S.filter_window(ff, n, p["B"]))
+        del ff
+        draws.append(S.draw_permutations(n, p["loops"], p["RLFC_loops"], p["W_RLFC"], 500 + i))
+    torch.cuda.synchronize()
+    return prms, xs, fts, wins, draws
+
+
+def test_batch64_workload_equals_singles_and_oracle(S):
+    """The 64-signal uneven workload: eager batch and graph replays equal the single-signal
+    transforms bit for bit; one signal of every drawn size equals the CPU oracle's FP32 path."""
+    import torch
+    prms, xs, fts, wins, draws = _workload64(S, seed=4)
+    qs = [p["n"].bit_length() - 1 for p in prms]
+    assert min(qs) >= 18 and max(qs) <= 24 and len(set(qs)) >= 5
+    assert all(50 <= p["k"] <= 4000 for p in prms)
+    assert all(p["B_thresh"] < p["B"] and p["B_thresh"] < p["W_RLFC"] for p in prms)
+    want = _singles(S, prms, xs, fts, wins, draws, "fp32")
+    b = S.Batch(prms, "fp32")
+    for i in range(len(prms)):
+        b.set_filter(i, fts[i], wins[i])
+    _same(b.execute(xs, draws), want)
+    b.set_graph(True)
+    for _ in range(2):
+        _same(b.execute(xs, draws), want)
+        (torch.randn(1 << 20, device="cuda") * 2).sum().item()
+        torch.cuda.synchronize()
+    b.set_graph(False)
+    res = b.execute(xs, draws)
+    _same(res, want)
+    seen = set()
+    for i, p in enumerate(prms):
+        q = qs[i]
+        if q in seen:
+            continue
+        seen.add(q)
+        a, ai, off = draws[i]
+        ref = O.outer_loop32(p, xs[i].cpu().numpy(), fts[i].cpu().numpy(), wins[i], a, ai, off)
+        nodes, cnt = res[i]
+        assert cnt == ref["I_F"], f"signal {i} (q={q})"
+        assert nodes.tobytes() == ref["nodes"].tobytes(), f"signal {i} (q={q})"
+    b.close()

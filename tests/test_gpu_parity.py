@@ -446,3 +446,40 @@ def test_comb_keys_bitexact(S, cfg):
         y = x[int(offs[r])::sigma][:W]
         want = O.samples64(O.fft64(y))
         assert np.array_equal(keys[r], want.view(np.uint64)), r
+
+
+def test_graph_plan_filter_swap(S):
+    """A graph-mode plan whose filter is replaced between replays: every transform equals an
+    eager plan's with the same filter, bit for bit, and the oracle's.  Each swap hands over a
+    fresh taps tensor while the previous one is dropped and its memory reused by other
+    allocations (the captured graph must not keep reading the old taps)."""
+    import torch
+    prm = params(SMALL_CONFIGS[1])
+    n, B, w = prm["n"], prm["B"], prm["w"]
+    x, locs, _ = sparse_signal(n, prm["k"], 303)
+    ft1, ff1 = oracle_filter(prm)
+    win1 = O.filter_window_slice(ff1, n, B)
+    rng = np.random.default_rng(304)
+    ft2 = ft1 * (1 + 1e-3 * rng.standard_normal(w))
+    win2 = win1 * (1 + 1e-3 * rng.standard_normal(win1.size))
+    a, ai, offs = draws(n, prm["loops"], prm["RLFC_loops"], prm["W_RLFC"], 305)
+    dx = _dev(x)
+    G = S.Plan(prm, "fp64")
+    G.set_graph(True)
+    E = S.Plan(prm, "fp64")
+    refs = {}
+    for it, (ft, win) in enumerate([(ft1, win1), (ft2, win2), (ft1, win1), (ft2, win2)]):
+        G.set_filter(_dev(ft), win)
+        E.set_filter(_dev(ft), win)
+        junk = [torch.full((w,), complex(it, -it), dtype=torch.complex128, device="cuda")
+                for _ in range(4)]
+        for _ in range(2):
+            gn, gc = G.execute(dx, a, ai, offs)
+            en, ec = E.execute(dx, a, ai, offs)
+            assert gc == ec and gn.tobytes() == en.tobytes(), f"swap {it}"
+        key = it % 2
+        if key not in refs:
+            refs[key] = O.outer_loop64(prm, x, ft, win, a, ai, offs)
+        assert gc == refs[key]["I_F"] and gn.tobytes() == refs[key]["nodes"].tobytes()
+        del junk
+    assert refs[0]["nodes"].tobytes() != refs[1]["nodes"].tobytes()

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from helpers import bits
 
 
 def _int_complex(rng, n, lo=-8, hi=9):
@@ -80,15 +81,107 @@ def test_perm_filter32_kahan_bound():
     assert np.all(np.abs(got - exact) <= 4 * eps * mag + 1e-30)
 
 
-def test_samples_fma():
+def test_samples_rounding():
+    """|X|^2 = re*re + im*im with each square rounded, then added (the reference's
+    pow(re,2) + pow(im,2), innerLoop.cu:715): bit for bit numpy's elementwise form, which does
+    exactly that.  A fused form (one rounding) differs from it on random inputs, so this pins
+    the choice."""
     rng = np.random.default_rng(4)
     X = _int_complex(rng, 1000, -1000, 1000)
     assert np.array_equal(O.samples64(X), X.real ** 2 + X.imag ** 2)   # exact for integers
-    Y = rng.standard_normal(1000) + 1j * rng.standard_normal(1000)
+    Y = rng.standard_normal(4000) + 1j * rng.standard_normal(4000)
     s = O.samples64(Y)
+    assert np.array_equal(s, Y.real * Y.real + Y.imag * Y.imag)
     re, im = Y.real.astype(np.longdouble), Y.imag.astype(np.longdouble)
-    exact = re * re + im * im                          # 64-bit mantissa: exact enough
-    assert np.max(np.abs((s - exact) / exact)) <= 2.3e-16   # fma(re, re, im*im): <= 1 ulp
+    fused = (re * re + np.longdouble(1) * (Y.imag * Y.imag)).astype(np.float64)
+    assert np.count_nonzero(fused != s) > 100          # the test tells the two forms apart
+    Z = Y.astype(np.complex64)
+    assert np.array_equal(O.samples32(Z), Z.real * Z.real + Z.imag * Z.imag)
+
+
+def _pf_sequential(x, n, f, B, ai, tiled, fp32=False):
+    """The fold as the reference writes it (innerLoop.cu:740-741, 770-771), restated in numpy:
+    per tap the term x.x*f.x - x.y*f.y (products rounded, then combined), added to the bucket
+    in ascending tap order; FP32 sums are Kahan-compensated.  Vectorised over buckets, so it
+    is exact for any inputs, not only exactly representable ones."""
+    w = f.size
+    dt = np.float32 if fp32 else np.float64
+    xr, xi = x.real.astype(dt), x.imag.astype(dt)
+    fr, fi = f.real.astype(dt), f.imag.astype(dt)
+    r = np.arange(B, dtype=np.int64)
+    sx, sy = np.zeros(B, dt), np.zeros(B, dt)
+    cx, cy = np.zeros(B, dt), np.zeros(B, dt)
+
+    def add(ids, rows):
+        j = (ids * ai) % n
+        tx = xr[j] * fr[ids] - xi[j] * fi[ids]
+        ty = xr[j] * fi[ids] + xi[j] * fr[ids]
+        if not fp32:
+            sx[rows] = sx[rows] + tx
+            sy[rows] = sy[rows] + ty
+            return
+        for s_, c_, t_ in ((sx, cx, tx), (sy, cy, ty)):
+            yv = t_ - c_[rows]
+            tv = s_[rows] + yv
+            c_[rows] = (tv - s_[rows]) - yv
+            s_[rows] = tv
+
+    if tiled:
+        tiles, residue = w // B, w % B
+        for t in range(tiles):
+            add(t * B + r, r)
+        rr = r[:residue]
+        add(rr * tiles + B, rr)
+    else:
+        for t in range(-(-w // B)):
+            ids = t * B + r
+            ok = ids < w
+            add(ids[ok], r[ok])
+    return sx + 1j * sy if not fp32 else (sx + 1j * sy).astype(np.complex64)
+
+
+@pytest.mark.parametrize("mode", [O.FOLD_REFERENCE, O.FOLD_ALL_TAPS])
+def test_perm_filter_term_then_add_random_inputs(mode):
+    rng = np.random.default_rng(21 + mode)
+    n, B, w = 1 << 14, 128, 5000
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    f = rng.standard_normal(w) + 1j * rng.standard_normal(w)
+    ai = int(rng.integers(0, n // 2)) * 2 + 1
+    tiled = mode == O.FOLD_REFERENCE
+    got = O.perm_filter64(x, n, f, B, ai, mode)
+    assert np.array_equal(bits(got), bits(_pf_sequential(x, n, f, B, ai, tiled)))
+    x32, f32 = x.astype(np.complex64), f.astype(np.complex64)
+    got32 = O.perm_filter32(x32, n, f32, B, ai, mode)
+    assert np.array_equal(bits(got32), bits(_pf_sequential(x32, n, f32, B, ai, tiled, True)))
+
+
+def test_estimate_values_rounding_random_bins():
+    """Estimation divides as innerLoop.cu:848-852 writes it: |F|^2 = F.x*F.x + F.y*F.y and
+    bins*conj(F) with products rounded and combined, then the lower median (Q4); restated in
+    numpy on random (non-exact) bins and filter values, bit for bit."""
+    rng = np.random.default_rng(12)
+    n, B, loops = 1 << 12, 64, 7
+    m = n // B
+    win = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    ai = (rng.integers(0, n // 2, loops) * 2 + 1).astype(np.int32)
+    b = rng.standard_normal((loops, B)) + 1j * rng.standard_normal((loops, B))
+    I = rng.choice(n, 50, replace=False).astype(np.int32)
+    got = O.estimate_values(I, b, loops, n, ai, B, win)
+    med = (loops - 1) // 2
+    for h, Iv in enumerate(I):
+        vr, vi = np.empty(loops), np.empty(loops)
+        for l in range(loops):
+            p = (int(ai[l]) * int(Iv)) % n
+            hh, d = p // m, p % m
+            if d > m // 2:
+                hh, d = (hh + 1) % B, d - m
+            F = win[((n - d) % n + m // 2) % n]
+            bb = b[l, hh]
+            a2 = F.real * F.real + F.imag * F.imag
+            vr[l] = (bb.real * F.real + bb.imag * F.imag) / a2
+            vi[l] = (bb.imag * F.real - bb.real * F.imag) / a2
+        assert got[h]["key"] == Iv
+        assert got[h]["re"] == np.sort(vr)[med] and got[h]["im"] == np.sort(vi)[med]
 
 
 def _cutoff_brute(s, K):
