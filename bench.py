@@ -1,17 +1,25 @@
-"""Benchmark: one sparse FFT (outer_loop) at n=2^27, k=16000 on one MI355X.
+"""Benchmark: one sparse FFT (outer_loop) at n=2^27, k=16000 on one MI355X, FP64 and FP32.
 
 A step is one whole transform of a device-resident signal: comb restriction, permute+filter
 of every loop, bucket FFTs, cutoffs, reverse hash and estimation, ending when Node[I_F] is in
 host memory (the reference's timed region, outerLoop.cu:65-190, without its per-call
-mallocs).  The signal, filter and plan are prepared before the timed region.
+mallocs).  The signal, filter and plan are prepared before the timed region.  The timed steps
+cycle through several permutation / comb-offset draws (how the comb loops' offsets fall
+relative to each other changes the comb stage's line sharing); the median and the worst draw
+are reported beside the mean.
 
-Prints one JSON line.  Extra objects: roofline (dominant kernel = permute+filter, algorithmic
-bytes / its event-timed duration), cpu_baseline (the oracle's sequential transform on a
-bounded sample of the same workload), accuracy (missed locations, MSE vs the FP64 truth).
+Prints one JSON line.  value = the FP64 path's mean ms per transform (the reference's
+precision); the FP32 path (complex64 signal, FP64 filter design and estimation) is timed in
+the same run ("fp32": ms, speed-up, error against the FP64 path).  Extra objects: roofline
+(the dominant kernel's algorithmic bytes / its event-timed duration, with the 64-byte gather
+granule of the algorithmic-bytes formula, and its counter traffic from profiles/),
+cpu_baseline (the oracle's sequential transform on the host cores), accuracy (missed
+locations, MSE against the FP64 truth spectrum).
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -21,10 +29,12 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-sfft_amd"))
 METRIC = ("ms per SFFT at n=2^27,k=16000 (FP32 vs FP64, MSE≤1e-4); % HBM peak")
 PUBLISHED_MS = 172.697          # README.md:102 (Titan Xp, same parameters, FP64)
 HBM_PEAK_GBS = 8000.0
-# bytes one random 8- or 16-byte gather moves from HBM: MI355X issues every read request to
-# memory as one 128-byte line (a 1 GiB copy makes 2^23 requests, and the permute+filter
-# kernel makes one request per tap it gathers).
-GATHER_GRANULE = 128
+# Bytes a random 8- or 16-byte gather is charged in the algorithmic-bytes formula (one 64-byte
+# HBM access granule).  The counters show every read request to memory on this device is a
+# whole 128-byte line (profiles/random_gather_granule.md), so the same kernels are also priced
+# at 128 bytes per gather (roofline.achieved_granule128).
+GATHER_GRANULE = 64
+LINE = 128
 
 CONFIGS = {
     # name: (q, k, Bcst, loops_loc, loops_thresh, loops_est, RLFC_cst, RLFC_loops, snr_db)
@@ -39,34 +49,238 @@ CONFIGS = {
 }
 BATCH = dict(count=64, q_range=(18, 24), k_range=(50, 4000), Bcst=1.0, loops_loc=2,
              loops_thresh=2, loops_est=5, RLFC_cst=256, RLFC_loops=2, snr_db=40.0)
+FRONT_KERNEL = {"columns": ("fold_slab_kernel", "stream"),
+                "gather": ("perm_filter_kernel", "fold"),
+                "stream": ("stream_scatter_kernel", "stream")}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="n27_k16000", choices=sorted(CONFIGS),
                     help="workload: n27_k16000 (default; n=2^27, k=16000), n20_k1000, "
                          "n27_k40000_30db (30 dB noise), batch64 (64 uneven signals), "
                          "n28_k16000 / n29_k16000 (the n27_k16000 constants at larger n)")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--front-end", default="auto", choices=["auto", "gather", "stream"],
-                    help="signal access: gathers per tap (auto), or one streaming pass")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "fp32"],
+                    help="both (default): the FP64 path (value) and the FP32 path")
+    ap.add_argument("--front-end", default="auto",
+                    choices=["auto", "columns", "gather", "stream"],
+                    help="signal access: auto (the plan's choice: column-ordered fold for "
+                         "FP32, gathers for FP64), columns, gathers in bucket order, or one "
+                         "streaming pass")
+    ap.add_argument("--draws", type=int, default=8,
+                    help="permutation / comb-offset draw sets the timed steps cycle through")
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
                     help="budget for the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=0x5FF70002)
     ap.add_argument("--design", default="faithful", choices=["faithful", "parallel"],
                     help="filter design mode (setup, outside the timed region): the "
                          "reference's rounding, or the device design without recurrences")
     ap.add_argument("--launch", default="graph", choices=["graph", "eager"],
-                    help="replay each transform from a hipGraph captured once (the default: "
-                         "7-12 us per transform faster at n=2^27, k=16000), or enqueue its "
-                         "kernels eagerly")
+                    help="replay each transform from a hipGraph captured once (the default), "
+                         "or enqueue its kernels eagerly")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB before the transforms "
                          "(a known-size kernel that calibrates HBM byte counters)")
+    ap.add_argument("--gather-probe", action="store_true",
+                    help="run random-gather probes (torch index_select of 16-byte elements: "
+                         "one per random line, then two per line) for the counter granule "
+                         "evidence, and exit")
     return ap.parse_args()
+
+
+def host_info():
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
+
+
+def gather_probe():
+    """Known-size random gathers for the HBM granule: N 16-byte elements read from N random
+    128-byte lines of a 4 GiB array, then N elements read as N/2 pairs sharing a line.  Run it
+    under rocprofv3 --pmc FETCH_SIZE: bytes per gathered element = FETCH_SIZE / N."""
+    import torch
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    nel = 1 << 28                                   # complex128 elements, 4 GiB
+    src = torch.zeros(nel, dtype=torch.complex128, device="cuda")
+    N = 1 << 22
+    g = torch.Generator(device="cuda").manual_seed(7)
+    lines = torch.randperm(nel // 8, device="cuda", generator=g)[:N]
+    one = (lines * 8).contiguous()                                   # one element per line
+    pairs = (lines[: N // 2].repeat_interleave(2) * 8 +
+             torch.tensor([0, 1], device="cuda").repeat(N // 2)).contiguous()
+    torch.cuda.synchronize()
+    for name, idx, nl in (("one_per_line", one, N), ("two_per_line", pairs, N // 2)):
+        out = torch.index_select(src, 0, idx)
+        torch.cuda.synchronize()
+        log(f"[probe] {name}: {N} gathers of 16 B from {nl} lines, output "
+            f"{out.numel() * 16} bytes")
+    print(json.dumps({"probe": "random_gather", "elements": N, "element_bytes": 16}))
+
+
+def run_precision(args, S, prm, prec, seeds_draws, snr, log):
+    """Inputs, plan, K timed steps cycling the draw sets, per-stage events, accuracy."""
+    import numpy as np
+    import torch
+    n, B, w, L, k = prm["n"], prm["B"], prm["w"], prm["loops"], prm["k"]
+    fp32 = prec == "fp32"
+    t = time.time()
+    locs, vals = S.signal_spectrum(n, k, args.seed)
+    x = S.synthesize(n, k, locs, vals, snr_db=snr or 0.0, noise_seed=args.seed + 1, fp32=fp32)
+    X = S.truth_spectrum(x)
+    log(f"[bench] {prec}: signal + truth spectrum {time.time() - t:.2f}s")
+    t = time.time()
+    ft, ff = S.make_filter(prm["lobefrac"], prm["tolerance"], n, prm["b_f"], w,
+                           mode=args.design, freq="device")
+    win = S.filter_window(ff, n, B)
+    del ff
+    if fp32:
+        ft = S.to_complex64(ft)
+    log(f"[bench] {prec}: filter design (FP64, {args.design}, w={w}) {time.time() - t:.2f}s")
+    # the plan's AUTO choice: column front end for FP32, gathers for FP64
+    front = args.front_end if args.front_end != "auto" else ("columns" if fp32 else "gather")
+    plan = S.Plan(prm, prec, front_end=front)
+    plan.set_filter(ft, win)
+    if args.launch == "graph":
+        plan.set_graph(True)
+    D = len(seeds_draws)
+    for i in range(max(args.warmup, 1)):
+        plan.execute(x, *seeds_draws[i % D], copy=False)
+    torch.cuda.synchronize()
+    step_ms = []
+    t_all = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        plan.execute(x, *seeds_draws[i % D], copy=False, sync_inputs=False)
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t_all) * 1e3 / args.steps
+    per_draw = [float(np.median(step_ms[d::D])) for d in range(min(D, args.steps))]
+    # stage breakdown: the same transforms run eagerly with hipEvents at the stage boundaries
+    plan.set_profiling(True)
+    stage_sum = iv_sum = None
+    for d in range(D):
+        plan.execute(x, *seeds_draws[d], copy=False, sync_inputs=False)
+        st, iv = plan.stage_times(), plan.interval_times()
+        stage_sum = st if stage_sum is None else {kk: stage_sum[kk] + v for kk, v in st.items()}
+        iv_sum = iv if iv_sum is None else {kk: iv_sum[kk] + v for kk, v in iv.items()}
+    plan.set_profiling(False)
+    stages_ms = {kk: v * 1e3 / D for kk, v in stage_sum.items()}
+    iv_ms = {kk: v * 1e3 / D for kk, v in iv_sum.items()}
+    nodes, cnt = plan.execute(x, *seeds_draws[0], copy=True)
+    acc = S.metric(nodes, k, locs, X)
+    log(f"[bench] {prec}: {args.steps} steps {ms:.3f} ms/step over {D} draw sets (per-set "
+        f"medians {min(per_draw):.3f}..{max(per_draw):.3f}, {args.launch}); stages(ms) "
+        + " ".join(f"{kk}={v:.3f}" for kk, v in stages_ms.items()))
+    log(f"[bench] {prec}: I_F={cnt} missed={acc['missed_est']},{acc['missed_res']} "
+        f"MSE*k={acc['mse_k']:.4e} MSE={acc['mse']:.4e}")
+    res = dict(ms=ms, step_ms=step_ms, per_draw=per_draw, stages_ms=stages_ms, iv_ms=iv_ms,
+               nodes=nodes, cnt=cnt, acc=acc, front=front, x=x, ft=ft, win=win)
+    del X
+    return res
+
+
+def kernel_roofline(prm, prec, front, iv_ms, config):
+    """The dominant kernel's rate: algorithmic bytes per launch (inputs read once, outputs
+    written once; a gathered tap charged one 64-byte granule, at most the whole signal) over
+    its event-timed duration."""
+    n, B, w, L = prm["n"], prm["B"], prm["w"], prm["loops"]
+    e = 8 if prec == "fp32" else 16
+    Wc, Rl = prm["W_RLFC"], prm["RLFC_loops"]
+    kname, iv = FRONT_KERNEL[front]
+    k_ms = iv_ms[iv]
+    if front == "stream":
+        b64 = b128 = n * e + L * w * e + Rl * Wc * e
+    else:
+        b64 = min(L * w * GATHER_GRANULE, n * e) + w * e + L * B * e
+        b128 = min(L * w * LINE, n * e) + w * e + L * B * e
+    achieved = b64 / (k_ms * 1e-3) / 1e9
+    traffic, tsrc = None, None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            entry = json.load(fh).get(f"{kname}_{config}_{prec}")
+        if isinstance(entry, dict):
+            traffic, tsrc = entry.get("bytes"), entry.get("source")
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": tsrc, "kernel": kname, "bytes_per_launch": b64,
+            "launch_ms": round(k_ms, 4), "granule_bytes": GATHER_GRANULE,
+            "achieved_granule128": round(b128 / (k_ms * 1e-3) / 1e9, 1),
+            "frac_granule128": round(b128 / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def transform_roofline(prm, prec, ms, cnt):
+    """Whole transform: ALG = min(RLFC_loops*W*g_c + loops*w*G, n*e) + w*e + I_F*24 with
+    G = 64 and g_c = G if sigma*e >= G else sigma*e (24-byte nodes in both precisions)."""
+    n, w, L = prm["n"], prm["w"], prm["loops"]
+    e = 8 if prec == "fp32" else 16
+    sig = n // prm["W_RLFC"]
+    g_c = GATHER_GRANULE if sig * e >= GATHER_GRANULE else sig * e
+    x_bytes = min(prm["RLFC_loops"] * prm["W_RLFC"] * g_c + L * w * GATHER_GRANULE, n * e)
+    alg = x_bytes + w * e + cnt * 24
+    rate = alg / (ms * 1e-3) / 1e9
+    return {"alg_bytes": alg, "GB/s": round(rate, 1), "frac": round(rate / HBM_PEAK_GBS, 4)}
+
+
+def node_error(a, b):
+    """FP32 nodes a against FP64 nodes b of the same signal and draws: the located sets, the
+    largest value difference on the common indices, and sum |a - b|^2 over the union (a node
+    missing on one side counts its whole value)."""
+    import numpy as np
+    va = {int(k_): complex(r, i) for k_, r, i in zip(a["key"], a["re"], a["im"])}
+    vb = {int(k_): complex(r, i) for k_, r, i in zip(b["key"], b["re"], b["im"])}
+    common = va.keys() & vb.keys()
+    diff = [abs(va[j] - vb[j]) for j in common]
+    err = sum(abs(va.get(j, 0) - vb.get(j, 0)) ** 2 for j in va.keys() | vb.keys())
+    return {"same_located_set": va.keys() == vb.keys(), "only_fp32": len(va.keys() - vb.keys()),
+            "only_fp64": len(vb.keys() - va.keys()),
+            "max_abs": float(max(diff)) if diff else 0.0, "sum_sq_diff": float(err),
+            "rms_common": float(np.sqrt(np.mean(np.square(diff)))) if diff else 0.0}
+
+
+def cpu_baseline(args, prm, res, seeds_draws, log):
+    """The oracle's sequential FP64 transform (gcc -O3, one thread) on the same signal, filter
+    and first draw set: one warm-up run, then the median of up to three within the budget."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    xh = res["x"].cpu().numpy().astype(np.complex128)
+    fth = res["ft"].cpu().numpy().astype(np.complex128)
+    a, ai, offs = seeds_draws[0]
+    t0 = time.perf_counter()
+    O.outer_loop64(prm, xh, fth, res["win"], a, ai, offs)            # warm-up
+    times = []
+    warm = time.perf_counter() - t0
+    while len(times) < 3 and time.perf_counter() - t0 + warm <= args.cpu_seconds:
+        t1 = time.perf_counter()
+        O.outer_loop64(prm, xh, fth, res["win"], a, ai, offs)
+        times.append(time.perf_counter() - t1)
+    if not times:
+        times = [warm]
+    info = host_info()
+    v = round(1e3 * float(np.median(times)), 1)
+    log(f"[bench] cpu baseline: {v} ms/transform (median of {len(times)} after a warm-up; "
+        f"{info['cpu_model']}, nproc {info['nproc']})")
+    return {"value": v, "unit": "ms", "cores": 1, "kind": "port",
+            "sample": f"{len(times)} full {args.config} FP64 transforms after one warm-up "
+                      f"(median), same signal, filter and first draw set (oracle "
+                      f"outer_loop64, gcc -O3 -ffp-contract=off, 1 thread)",
+            "cpu_model": info["cpu_model"], "nproc": info["nproc"],
+            "usable_cpus": info["usable_cpus"]}
 
 
 def main():
@@ -75,43 +289,25 @@ def main():
         print("bench.py: this build measures one GPU only (the transform is not sharded); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
+    import numpy as np  # noqa: F401
     import torch
 
     import sfft_amd as S
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.gather_probe:
+        return gather_probe()
     if args.config == "batch64":
         return main_batch(args)
     q, k, Bcst, ll, lt, le, rc, rl, snr = CONFIGS[args.config]
     prm = S.derive_params(q, k, Bcst, ll, lt, le, rc, rl)
     n, B, w, L = prm["n"], prm["B"], prm["w"], prm["loops"]
-    fp32 = args.precision == "fp32"
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
-    log(f"[bench] {args.config} n={n} k={k} B={B} W={prm['W_RLFC']} w={w} loops={L} "
-        f"prec={args.precision}")
-
-    # ---- inputs (outside the timed region) ----
-    t = time.time()
-    locs, vals = S.signal_spectrum(n, k, args.seed)
-    x = S.synthesize(n, k, locs, vals, snr_db=snr or 0.0, noise_seed=args.seed + 1, fp32=fp32)
-    X = S.truth_spectrum(x)
-    log(f"[bench] signal + truth spectrum: {time.time() - t:.2f}s")
-    t = time.time()
-    ft, ff = S.make_filter(prm["lobefrac"], prm["tolerance"], n, prm["b_f"], w,
-                           mode=args.design, freq="device")
-    win = S.filter_window(ff, n, B)
-    del ff
-    if fp32:
-        ft = S.to_complex64(ft)
-    log(f"[bench] filter design (FP64, {args.design}, w={w}): {time.time() - t:.2f}s")
-    a, ai, offs = S.draw_permutations(n, L, prm["RLFC_loops"], prm["W_RLFC"], args.seed + 2)
-
-    front = "gather" if args.front_end == "auto" else args.front_end
-    plan = S.Plan(prm, args.precision, front_end=front)
-    plan.set_filter(ft, win)
-    if args.launch == "graph":
-        plan.set_graph(True)
+    log(f"[bench] {args.config} n={n} k={k} B={B} W={prm['W_RLFC']} w={w} loops={L}")
+    seeds_draws = [S.draw_permutations(n, L, prm["RLFC_loops"], prm["W_RLFC"],
+                                       args.seed + 2 + 17 * d) for d in range(args.draws)]
+    log("[bench] comb offsets per draw set: " + " ".join(
+        "/".join(str(int(o)) for o in dr[2]) for dr in seeds_draws))
 
     if args.copy_probe_mib > 0:
         nb = args.copy_probe_mib << 20
@@ -123,121 +319,63 @@ def main():
         log(f"[bench] copy probe: {nb} bytes read, {nb} bytes written")
         del src, dst
 
-    # ---- warm-up, then K timed transforms (Node[I_F] host-visible at the end of each) ----
-    for _ in range(args.warmup):
-        nodes, cnt = plan.execute(x, a, ai, offs, copy=False)
-    torch.cuda.synchronize()
-    step_ms = []
-    t_all = time.perf_counter()
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        nodes, cnt = plan.execute(x, a, ai, offs, copy=False, sync_inputs=False)
-        step_ms.append((time.perf_counter() - t0) * 1e3)
-    torch.cuda.synchronize()
-    total_s = time.perf_counter() - t_all
-    ms_per_step = total_s * 1e3 / args.steps
+    precs = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
+    runs = {}
+    for prec in precs:
+        runs[prec] = run_precision(args, S, prm, prec, seeds_draws, snr, log)
+        torch.cuda.synchronize()
+    main_prec = precs[0]
+    R = runs[main_prec]
+    cpu = cpu_baseline(args, prm, R, seeds_draws, log) if args.cpu_seconds > 0 else None
 
-    # ---- stage breakdown: the same transform run eagerly with hipEvents at the stage
-    # boundaries (the graph replays the same kernels) ----
-    plan.set_profiling(True)
-    prof_steps = max(3, min(args.steps, 10))
-    stage_sum = iv_sum = None
-    for _ in range(prof_steps):
-        plan.execute(x, a, ai, offs, copy=False, sync_inputs=False)
-        st = plan.stage_times()
-        stage_sum = st if stage_sum is None else {kk: stage_sum[kk] + v for kk, v in st.items()}
-        iv = plan.interval_times()
-        iv_sum = iv if iv_sum is None else {kk: iv_sum[kk] + v for kk, v in iv.items()}
-    plan.set_profiling(False)
-    stages_ms = {kk: v * 1e3 / prof_steps for kk, v in stage_sum.items()}
-    iv_ms = {kk: v * 1e3 / prof_steps for kk, v in iv_sum.items()}
-    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (median "
-        f"{float(np.median(step_ms)):.3f}, {args.launch}); "
-        f"stages(ms, eager + events) "
-        + " ".join(f"{kk}={v:.3f}" for kk, v in stages_ms.items()))
+    def summary(prec):
+        r = runs[prec]
+        return {"ms": round(r["ms"], 4), "median_draw_ms": round(float(np.median(r["per_draw"])), 4),
+                "worst_draw_ms": round(max(r["per_draw"]), 4),
+                "per_draw_ms": [round(v, 4) for v in r["per_draw"]],
+                "stages_ms": {kk: round(v, 4) for kk, v in r["stages_ms"].items()},
+                "intervals_ms": {kk: round(v, 4) for kk, v in r["iv_ms"].items()},
+                "roofline": kernel_roofline(prm, prec, r["front"], r["iv_ms"], args.config),
+                "transform_roofline": transform_roofline(prm, prec, r["ms"], r["cnt"]),
+                "accuracy": {"I_F": r["cnt"],
+                             "missed": [r["acc"]["missed_est"], r["acc"]["missed_res"]],
+                             "mse_k": r["acc"]["mse_k"], "mse": r["acc"]["mse"],
+                             "max_abs": r["acc"]["max_abs"]}}
 
-    acc = S.metric(nodes, k, locs, X)
-    log(f"[bench] I_F={cnt} missed={acc['missed_est']},{acc['missed_res']} "
-        f"MSE*k={acc['mse_k']:.4e} MSE={acc['mse']:.4e}")
-
-    # ---- roofline of the dominant kernel ----
-    # gather front end: permute+filter, L*w random 64-byte granules + taps + bins.
-    # streaming front end: the streaming pass, reads x once, writes the L*w tap rows and the
-    # Rl*W comb samples.
-    e = 8 if fp32 else 16
-    Wc, Rl = prm["W_RLFC"], prm["RLFC_loops"]
-    if front == "stream":
-        kname, k_bytes, k_ms = ("stream_scatter_kernel", n * e + L * w * e + Rl * Wc * e,
-                                iv_ms["stream"])
-    else:
-        kname, k_bytes, k_ms = ("perm_filter_kernel", L * w * GATHER_GRANULE + w * e + L * B * e,
-                                iv_ms["fold"])
-    achieved = k_bytes / (k_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get(f"{kname}_{args.config}_{args.precision}")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": kname, "bytes_per_launch": k_bytes, "launch_ms": round(k_ms, 4),
-                "granule_bytes": GATHER_GRANULE if front == "gather" else None}
-    if front == "gather":
-        # the same kernel priced with a 64-byte granule (half a line per gather)
-        b64 = L * w * 64 + w * e + L * B * e
-        roofline["achieved_granule64"] = round(b64 / (k_ms * 1e-3) / 1e9, 1)
-    # whole-transform algorithmic rate: one 128-B line per gathered tap and comb sample (at most
-    # the whole signal), the taps, and the nodes (24-byte FP64 nodes in both precisions)
-    sig = n // prm["W_RLFC"]
-    g_c = GATHER_GRANULE if sig * e >= GATHER_GRANULE else sig * e
-    x_bytes = min(prm["RLFC_loops"] * prm["W_RLFC"] * g_c + L * w * GATHER_GRANULE, n * e)
-    alg = x_bytes + w * e + cnt * 24
-    transform = {"alg_bytes": alg, "GB/s": round(alg / (ms_per_step * 1e-3) / 1e9, 1),
-                 "frac": round(alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-
-    # ---- CPU baseline: the oracle's sequential transform, 1 core, bounded sample ----
-    cpu = None
-    if args.cpu_seconds > 0:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        xh = x.cpu().numpy().astype(np.complex128)
-        fth = ft.cpu().numpy().astype(np.complex128)
-        times = []
-        t_budget = time.perf_counter()
-        while True:
-            t0 = time.perf_counter()
-            O.outer_loop64(prm, xh, fth, win, a, ai, offs)
-            times.append(time.perf_counter() - t0)
-            if time.perf_counter() - t_budget + times[-1] > args.cpu_seconds or len(times) >= 5:
-                break
-        cpu = {"value": round(1e3 * float(np.mean(times)), 1), "unit": "ms", "cores": 1,
-               "kind": "port",
-               "sample": f"{len(times)} full {args.config} transforms of the same signal, "
-                         f"filter and draws (oracle outer_loop64, gcc -O2, 1 thread)"}
-        log(f"[bench] cpu baseline: {cpu['value']} ms/transform over {len(times)} runs")
-
+    main_sum = summary(main_prec)
     out = {
-        "metric": METRIC, "value": round(ms_per_step, 4), "unit": "ms", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "metric": METRIC, "value": round(R["ms"], 4), "unit": "ms", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(R["ms"], 4),
         "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": round(ms_per_step / PUBLISHED_MS, 6) if args.config == "n27_k16000" and
-        not fp32 else None,
-        "dtype": "f32" if fp32 else "f64", "data": "synthetic", "front_end": front,
-        "launch": "hipGraph" if args.launch == "graph" else "eager",
+        "vs_baseline": round(R["ms"] / PUBLISHED_MS, 6)
+        if args.config == "n27_k16000" and main_prec == "fp64" else None,
+        "dtype": "f32" if main_prec == "fp32" else "f64", "data": "synthetic",
+        "front_end": R["front"], "launch": "hipGraph" if args.launch == "graph" else "eager",
         "config": {"workload": f"{args.config}: sfft n=2^{q} k={k} Bcst={Bcst} "
                                f"loops={ll}+{le} thresh={lt} RLFC={rc}x{rl} "
                                f"{'noise ' + str(snr) + 'dB' if snr else 'noiseless'}, "
-                               f"{args.precision} path, {front} front end",
+                               f"{main_prec} path (value), {R['front']} front end, "
+                               f"{args.draws} draw sets",
                    "n": n, "k": k, "B": B, "W_RLFC": prm["W_RLFC"], "w": w, "loops": L,
-                   "filter_design": args.design, "parallelism": "none (1 GPU)"},
-        "median_ms": round(float(np.median(step_ms)), 4),
-        "stages_ms": {kk: round(v, 4) for kk, v in stages_ms.items()},
-        "intervals_ms": {kk: round(v, 4) for kk, v in iv_ms.items()},
-        "roofline": roofline, "transform_roofline": transform,
-        "accuracy": {"I_F": cnt, "missed": [acc["missed_est"], acc["missed_res"]],
-                     "mse_k": acc["mse_k"], "mse": acc["mse"], "max_abs": acc["max_abs"]},
+                   "draw_sets": args.draws, "filter_design": args.design,
+                   "parallelism": "none (1 GPU)"},
+        "median_draw_ms": main_sum["median_draw_ms"], "worst_draw_ms": main_sum["worst_draw_ms"],
+        "per_draw_ms": main_sum["per_draw_ms"],
+        "stages_ms": main_sum["stages_ms"], "intervals_ms": main_sum["intervals_ms"],
+        "roofline": main_sum["roofline"], "transform_roofline": main_sum["transform_roofline"],
+        "accuracy": main_sum["accuracy"],
         "cpu_baseline": cpu,
     }
+    if "fp32" in runs and main_prec == "fp64":
+        s32 = summary("fp32")
+        s32["speedup_vs_fp64"] = round(R["ms"] / runs["fp32"]["ms"], 4)
+        s32["error_vs_fp64_nodes"] = node_error(runs["fp32"]["nodes"], R["nodes"])
+        out["fp32"] = s32
+        out["fp64_ms"] = round(R["ms"], 4)
+        out["fp32_ms"] = s32["ms"]
+        out["fp32_speedup"] = s32["speedup_vs_fp64"]
+        log(f"[bench] fp32 / fp64: {s32['ms']:.4f} / {R['ms']:.4f} ms = "
+            f"{s32['speedup_vs_fp64']:.3f}x; nodes vs fp64 {s32['error_vs_fp64_nodes']}")
     print(json.dumps(out), flush=True)
 
 

@@ -86,7 +86,7 @@ constexpr int FFT_QMAX = 16;                 // columns per tile
 // or 256 (FP32: eight), so more gathers are in flight while other workgroups compute
 // (measured at n = 2^27: 256 threads cost FP64 10 us, 1024 cost it 65 us; FP32 gains 6 us).
 template <typename Real, int LOAD = 0>
-struct Tile {
+struct Tile {   // LOAD: kLoadPlain, kLoadComb (gathers from x) or kLoadCombRows (plain tile)
   static constexpr int cap = LOAD == 1 ? 2048 : (sizeof(Real) == 8 ? 4096 : 8192);
   // threads per workgroup
   static constexpr int nt = LOAD == 1 ? (sizeof(Real) == 8 ? 512 : 256) : 1024;
@@ -126,6 +126,7 @@ struct PassArgs {
   int nbatch;
   int keys_tiled;   // store keys at tile * TE + e (the tile's own contiguous range)
   uint16_t* digits; // optional first digits of the keys (same positions)
+  int log2nlo, log2m;   // kLoadCombRows: g at in[(g mod 2^log2nlo) << log2m | g >> log2nlo]
 };
 
 template <typename T2>
@@ -286,6 +287,8 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt)) void fft_pass_kernel(PassAr
       T2 val;
       if (LOAD == kLoadComb)
         val = ((const T2*)A.comb_x)[(long long)A.comb_offs[b] + g * (long long)A.comb_sigma];
+      else if (LOAD == kLoadCombRows)
+        val = in[((g & ((1LL << A.log2nlo) - 1)) << A.log2m) | (g >> A.log2nlo)];
       else
         val = in[g];
       if (!first && v > 0) val = cmul(val, outer_tw<T2>(A, (uint32_t)((t0 + c) * v), log2Lcur));
@@ -349,6 +352,13 @@ static hipError_t launch_pass(const PassArgs& A, int tiles, int batch, hipStream
 template <typename Real>
 static hipError_t dispatch_pass(const PassArgs& A, int load, int store, int tiles, int batch,
                                 hipStream_t st) {
+  if (load == kLoadCombRows) {
+    if (store == kStoreCplx)
+      return launch_pass<Real, kLoadCombRows, kStoreCplx>(A, tiles, batch, st);
+    if (store == kStoreKeys)
+      return launch_pass<Real, kLoadCombRows, kStoreKeys>(A, tiles, batch, st);
+    return launch_pass<Real, kLoadCombRows, kStoreBoth>(A, tiles, batch, st);
+  }
   if (load == kLoadComb) {
     if (store == kStoreCplx) return launch_pass<Real, kLoadComb, kStoreCplx>(A, tiles, batch, st);
     if (store == kStoreKeys) return launch_pass<Real, kLoadComb, kStoreKeys>(A, tiles, batch, st);
@@ -433,6 +443,8 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.comb_x = L.comb_x;
     A.comb_offs = L.comb_offs;
     A.comb_sigma = L.comb_sigma;
+    A.log2nlo = L.comb_log2nlo;
+    A.log2m = L.comb_log2m;
     A.nbatch = L.batch;
     A.keys_tiled = (last && p > 0 && L.keys_tiled) ? 1 : 0;
     A.digits = last ? L.digits : nullptr;
@@ -452,15 +464,22 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
 // ============================================================================================
 // The term as the reference writes it (innerLoop.cu:740-741, 770-771): both products
 // rounded, combined, then added to the bucket sum.
+// fold_term(x, f) is the tap's product, accumulate() adds a term: add() = both, in that order.
+__device__ __forceinline__ double2 fold_term(double2 x, double2 f) {
+  return make_double2(x.x * f.x - x.y * f.y, x.x * f.y + x.y * f.x);
+}
+__device__ __forceinline__ float2 fold_term(float2 x, float2 f) {
+  return make_float2(x.x * f.x - x.y * f.y, x.x * f.y + x.y * f.x);
+}
+
 struct Acc64 {
   double2 s;
   __device__ __forceinline__ void init() { s = make_double2(0.0, 0.0); }
-  __device__ __forceinline__ void add(double2 x, double2 f) {
-    const double tx = x.x * f.x - x.y * f.y;
-    const double ty = x.x * f.y + x.y * f.x;
-    s.x = s.x + tx;
-    s.y = s.y + ty;
+  __device__ __forceinline__ void accumulate(double2 t) {
+    s.x = s.x + t.x;
+    s.y = s.y + t.y;
   }
+  __device__ __forceinline__ void add(double2 x, double2 f) { accumulate(fold_term(x, f)); }
   __device__ __forceinline__ double2 get() const { return s; }
 };
 
@@ -473,12 +492,11 @@ struct Acc32 {  // Kahan-compensated FP32 fold
     comp = (t - sum) - y;
     sum = t;
   }
-  __device__ __forceinline__ void add(float2 x, float2 f) {
-    const float tx = x.x * f.x - x.y * f.y;
-    const float ty = x.x * f.y + x.y * f.x;
-    kahan(sx, cx, tx);
-    kahan(sy, cy, ty);
+  __device__ __forceinline__ void accumulate(float2 t) {
+    kahan(sx, cx, t.x);
+    kahan(sy, cy, t.y);
   }
+  __device__ __forceinline__ void add(float2 x, float2 f) { accumulate(fold_term(x, f)); }
   __device__ __forceinline__ float2 get() const { return make_float2(sx, sy); }
 };
 
@@ -669,6 +687,374 @@ hipError_t fold_taps_launch(int is_fp32, const void* T, const void* f, void* bin
                             const int* ai_dev, int n, int B, int w, int loops, int tiled,
                             hipStream_t st) {
   return pf_launch<true>(is_fp32, T, f, bins, ai_dev, n, B, w, loops, tiled, st);
+}
+
+// ============================================================================================
+// column-ordered fold
+//
+// View x as m = n/B rows of B columns (x[row*B + col]).  Every tap id = t*B + r of bucket r in
+// loop l gathers x[(id*ai_l) mod n], whose column is (id*ai_l) mod B = (r*ai_l) mod B: all the
+// taps of a bucket lie in ONE column, c = r*ai_l mod B, and conversely column c holds bucket
+// r = c*a_l mod B of loop l.  A workgroup that owns a block of adjacent columns for every loop
+// therefore gathers only from the 128-byte lines of those columns, and the taps of different
+// loops that land in the same line share one fetch (at n = 2^27, k = 16000: 20.1 M taps over
+// 8.4 M lines of complex64, 16.8 M of complex128).  The filter taps of bucket r are read from
+// a bucket-major copy fT[r*P + t] (one contiguous row per bucket, built by filter_rows_kernel
+// when the filter is set), since the buckets of one column block are scattered over r.
+// Each thread owns one (loop, column) pair and sums its bucket's taps in the canonical order
+// (t ascending, then the reference's tail tap when tiled), so the sums are the bits of
+// pf_body.
+// The comb samples y_r[j] = x[off_r + j*sigma] (sigma = n/W divides B) are the columns
+// c = off_r mod sigma + lo*sigma of the same view: sample j = row*(B/sigma) + lo.  The threads
+// of a column block also copy the comb columns that fall in it, while its lines are being
+// fetched for the taps, into column-major rows yc[r*W + lo*m + row] (m contiguous samples per
+// comb column); the comb FFT's first pass loads them with kLoadCombRows.
+// ============================================================================================
+constexpr int FC_NT = 256;
+constexpr int FC_CHUNK = 8;   // taps whose loads are issued together
+
+// fT[r*P + t]: row r holds the taps of bucket r in summation order: t*B + r for t < T, then at
+// t = T the tiled fold's tail tap r*T + B (tiled, r < w - T*B) or the last full-fold tap T*B + r
+// (r < w - T*B); zero elsewhere.  T = w / B.
+template <typename T2>
+__global__ void filter_rows_kernel(const T2* __restrict__ f, T2* __restrict__ fT, int B, int w,
+                                   int T, int P, int tiled) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)B * P) return;
+  const int r = (int)(i / P), t = (int)(i % P);
+  const int residue = w - T * B;
+  long long id = -1;
+  if (t < T) id = (long long)t * B + r;
+  else if (t == T && r < residue) id = tiled ? (long long)r * T + B : (long long)T * B + r;
+  T2 v;
+  v.x = 0;
+  v.y = 0;
+  fT[i] = (id >= 0) ? f[id] : v;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(FC_NT) void fold_columns_kernel(
+    const typename CT<Real>::T2* __restrict__ x, const typename CT<Real>::T2* __restrict__ fT,
+    typename CT<Real>::T2* __restrict__ bins, const int* __restrict__ a_dev,
+    const int* __restrict__ ai_dev, uint32_t mask, int log2B, int w, int loops, int tiled,
+    int T, int P, int log2lw, int remap, typename CT<Real>::T2* __restrict__ yc,
+    const int* __restrict__ offs, int Rl, int log2sigma, int log2m) {
+  using T2 = typename CT<Real>::T2;
+  using AC = typename AccOf<Real>::A;
+  const int blk = remap ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const uint32_t item = (uint32_t)blk * FC_NT + threadIdx.x;
+  const uint32_t B = 1u << log2B;
+  if (item >= (uint32_t)loops * B) return;
+  // item = ((column block * loops + l) << log2lw) | column within the block
+  const uint32_t lw = 1u << log2lw;
+  const uint32_t cc = item & (lw - 1u), rest = item >> log2lw;
+  const uint32_t l = rest % (uint32_t)loops, cb = rest / (uint32_t)loops;
+  const uint32_t c = (cb << log2lw) | cc;
+  const uint32_t ai = (uint32_t)ai_dev[l];
+  const uint32_t r = (c * (uint32_t)a_dev[l]) & (B - 1u);
+  const int residue = w - T * (int)B;
+  // main taps t*B + r: T of them, T+1 for the full fold's r < residue
+  const int nmain = (!tiled && (int)r < residue) ? T + 1 : T;
+  const uint32_t base = (r * ai) & mask, step = (B * ai) & mask;
+  const T2* __restrict__ frow = fT + (size_t)r * (size_t)P;
+  AC acc;
+  acc.init();
+  for (int t0 = 0; t0 < nmain; t0 += FC_CHUNK) {
+    T2 xv[FC_CHUNK], fv[FC_CHUNK];
+#pragma unroll
+    for (int k = 0; k < FC_CHUNK; ++k) {
+      const int t = t0 + k;
+      if (t < nmain) {
+        xv[k] = x[(base + (uint32_t)t * step) & mask];
+        fv[k] = frow[t];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < FC_CHUNK; ++k)
+      if (t0 + k < nmain) acc.add(xv[k], fv[k]);
+  }
+  if (tiled && (int)r < residue) {   // the reference's tail tap (Q2): another column
+    const uint32_t id = r * (uint32_t)T + B;
+    acc.add(x[(id * ai) & mask], frow[T]);
+  }
+  bins[(size_t)l * B + r] = acc.get();
+  // comb columns of this column block: rank-strided rows, contiguous stores
+  const uint32_t rank = (l << log2lw) | cc, G = (uint32_t)loops << log2lw;
+  const uint32_t c0 = cb << log2lw, smask = (1u << log2sigma) - 1u, m = 1u << log2m;
+  const size_t W = (size_t)B << log2m >> log2sigma;
+  for (int rr = 0; rr < Rl; ++rr) {
+    const uint32_t o = (uint32_t)offs[rr];
+    for (uint32_t col = c0 + ((o - c0) & smask); col < c0 + lw; col += smask + 1u) {
+      T2* __restrict__ dst = yc + (size_t)rr * W + ((size_t)(col >> log2sigma) << log2m);
+#pragma unroll 4
+      for (uint32_t row = rank; row < m; row += G) dst[row] = x[((size_t)row << log2B) | col];
+    }
+  }
+}
+
+// LDS-staged column fold.  A workgroup owns a slab of SC adjacent columns (SC * e = 256 bytes,
+// two 128-byte lines per row: narrower slabs read at a power-of-two row stride reach only half
+// the bandwidth on MI355X) over all m rows of the view, and streams it through LDS in chunks of
+// SL_ROWS rows (16-byte loads; the next chunk's loads are in flight in registers while the
+// current chunk is used): across the grid the whole signal is read once.  Each (loop, column)
+// thread first loads its bucket's filter row into its slots of an LDS tap array (slot t of
+// thread p at t * pairs + p); as the rows of its taps pass (tap t lies in row
+// ((r*ai + t*B*ai) mod n) / B) it replaces slot t by the tap's product term x * f; after the
+// last chunk it adds its slots up in the canonical order.  The comb columns of the slab are
+// copied out of each chunk.  Same per-bucket terms and order as pf_body: the same bits.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// complex element as a native vector (arrays of HIP's double2/float2 wrappers end up in scratch)
+template <typename Real>
+struct NV {
+  typedef Real type __attribute__((ext_vector_type(2)));
+};
+constexpr int SL_ROWS = 128;  // rows per chunk
+constexpr int SL_PITCH = 17;  // 16-byte units per chunk row: 256 bytes + 16 of padding (banks)
+constexpr int SL_U = 4;       // 16-byte loads per thread for one chunk (512 threads)
+
+template <typename Real, int PMAX, int NT>
+__global__ __launch_bounds__(NT) void fold_slab_kernel(
+    const typename CT<Real>::T2* __restrict__ x, const typename CT<Real>::T2* __restrict__ fT,
+    typename CT<Real>::T2* __restrict__ bins, const int* __restrict__ a_dev,
+    const int* __restrict__ ai_dev, uint32_t mask, int log2B, int log2m, int w, int loops,
+    int tiled, int T, int P, int log2sc, typename CT<Real>::T2* __restrict__ yc,
+    const int* __restrict__ offs, int Rl, int log2sigma) {
+  using T2 = typename CT<Real>::T2;
+  using V2 = typename NV<Real>::type;
+  using AC = typename AccOf<Real>::A;
+  constexpr int EPU = 16 / (int)sizeof(T2);             // elements per 16-byte unit
+  extern __shared__ u32x4 dyn16[];
+  u32x4* chunk16 = dyn16;                              // SL_ROWS x SL_PITCH units
+  const V2* ldsv = (const V2*)dyn16;                   // element (row, k): row*PITCH*EPU + k
+  V2* slotv = (V2*)(dyn16 + SL_ROWS * SL_PITCH);       // PMAX x pairs slots
+  const int tid = threadIdx.x;
+  const uint32_t B = 1u << log2B, m = 1u << log2m, SC = 1u << log2sc;
+  const uint32_t c0 = (uint32_t)blockIdx.x << log2sc;   // whole lines per row: no XCD remap
+  const int pairs = loops << log2sc;
+  const bool active = tid < pairs;
+  const uint32_t cc = (uint32_t)tid & (SC - 1u), l = (uint32_t)tid >> log2sc;
+  uint32_t r = 0, ai = 0, base = 0, step = 0;
+  int nmain = 0;
+  const int residue = w - T * (int)B;
+  const int rows = (int)(m < (uint32_t)SL_ROWS ? m : (uint32_t)SL_ROWS);
+  const int nchunks = (int)(m / (uint32_t)rows);
+  const int units = rows * 16;               // 16-byte units of x per chunk
+  const u32x4* __restrict__ x16 = (const u32x4*)x;
+  const size_t row16 = ((size_t)B * sizeof(T2)) / 16, col16 = ((size_t)c0 * sizeof(T2)) / 16;
+  const uint32_t smask = (1u << log2sigma) - 1u;
+  const size_t W = (size_t)B << log2m >> log2sigma;
+  u32x4 nx[SL_U];
+  auto load = [&](int ch) {
+#pragma unroll
+    for (int u = 0; u < SL_U; ++u) {
+      const int i = u * NT + tid;
+      if (u * NT < units && i < units)
+        nx[u] = __builtin_nontemporal_load(
+            &x16[((size_t)ch * rows + (size_t)(i >> 4)) * row16 + col16 + (size_t)(i & 15)]);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int u = 0; u < SL_U; ++u) {
+      const int i = u * NT + tid;
+      if (u * NT < units && i < units) chunk16[(i >> 4) * SL_PITCH + (i & 15)] = nx[u];
+    }
+  };
+  load(0);
+  // the filter row of the bucket into the slots while the first chunk is on its way
+  if (active) {
+    ai = (uint32_t)ai_dev[l];
+    r = ((c0 + cc) * (uint32_t)a_dev[l]) & (B - 1u);
+    nmain = (!tiled && (int)r < residue) ? T + 1 : T;
+    base = (r * ai) & mask;
+    step = (B * ai) & mask;
+    const V2* __restrict__ frow = (const V2*)(fT + (size_t)r * (size_t)P);
+#pragma unroll
+    for (int t = 0; t < PMAX; ++t)
+      if (t < nmain) slotv[t * pairs + tid] = frow[t];
+  }
+  store();
+  __syncthreads();
+#pragma unroll 1
+  for (int ch = 0; ch < nchunks; ++ch) {
+    if (ch + 1 < nchunks) load(ch + 1);
+    const uint32_t r0 = (uint32_t)(ch * rows);
+    if (active) {
+      // the taps whose rows fall in this chunk (a bit per tap): slot t becomes x * f
+      uint32_t hits = 0;
+#pragma unroll
+      for (int t = 0; t < PMAX; ++t) {
+        const uint32_t row = ((base + (uint32_t)t * step) & mask) >> log2B;
+        hits |= (t < nmain && row - r0 < (uint32_t)rows) ? (1u << t) : 0u;
+      }
+      while (hits) {
+        const int t = __ffs(hits) - 1;
+        hits &= hits - 1u;
+        const uint32_t row = ((base + (uint32_t)t * step) & mask) >> log2B;
+        const V2 xv = ldsv[(row - r0) * (SL_PITCH * EPU) + cc];
+        const V2 fv = slotv[t * pairs + tid];
+        T2 xt, ft;
+        xt.x = xv.x;
+        xt.y = xv.y;
+        ft.x = fv.x;
+        ft.y = fv.y;
+        const T2 term = fold_term(xt, ft);
+        V2 tv;
+        tv.x = term.x;
+        tv.y = term.y;
+        slotv[t * pairs + tid] = tv;
+      }
+    }
+    for (int rr = 0; rr < Rl; ++rr) {   // comb columns of the slab: this chunk's rows
+      const uint32_t o = (uint32_t)offs[rr];
+      for (uint32_t col = c0 + ((o - c0) & smask); col < c0 + SC; col += smask + 1u) {
+        V2* __restrict__ dst =
+            (V2*)yc + (size_t)rr * W + ((size_t)(col >> log2sigma) << log2m) + r0;
+        for (int row = tid; row < rows; row += NT)
+          dst[row] = ldsv[row * (SL_PITCH * EPU) + (col - c0)];
+      }
+    }
+    __syncthreads();
+    if (ch + 1 < nchunks) {
+      store();
+      __syncthreads();
+    }
+  }
+  if (!active) return;
+  AC acc;
+  acc.init();
+#pragma unroll
+  for (int t = 0; t < PMAX; ++t) {
+    if (t < nmain) {
+      const V2 tv = slotv[t * pairs + tid];
+      T2 term;
+      term.x = tv.x;
+      term.y = tv.y;
+      acc.accumulate(term);
+    }
+  }
+  if (tiled && (int)r < residue) {   // the reference's tail tap (Q2): another column
+    const uint32_t id = r * (uint32_t)T + B;
+    acc.add(x[(id * ai) & mask], fT[(size_t)r * (size_t)P + T]);
+  }
+  bins[(size_t)l * B + r] = acc.get();
+}
+
+int fold_rows_pitch(int B, int w) { return w / B + 1; }
+
+static size_t slab_lds_bytes(int is_fp32, int loops, int lsc, int P) {
+  return (size_t)SL_ROWS * SL_PITCH * 16 + (size_t)P * ((size_t)loops << lsc) * (is_fp32 ? 8 : 16);
+}
+
+// columns per slab (log2) for the LDS-staged fold, or -1 when it does not apply: 256-byte row
+// segments, at most 512 (loop, column) threads, at most 32 taps per bucket, at most 96 KB of
+// LDS (chunk + tap slots)
+static int slab_log2sc(int is_fp32, int log2B, int log2m, int loops, int P) {
+  const int lsc = is_fp32 ? 5 : 4;
+  (void)log2m;
+  if (lsc > log2B) return -1;
+  if ((loops << lsc) > 512 || P > 32) return -1;
+  if (slab_lds_bytes(is_fp32, loops, lsc, P) > 96 * 1024) return -1;
+  return lsc;
+}
+
+template <typename Real, int PMAX, int NT>
+static hipError_t slab_launch(const void* x, const void* fT, void* bins, const int* a_dev,
+                              const int* ai_dev, uint32_t mask, int log2B, int log2m, int w,
+                              int loops, int tiled, int T, int P, int lsc, void* yc,
+                              const int* offs, int Rl, int log2sigma, hipStream_t st) {
+  using T2 = typename CT<Real>::T2;
+  const unsigned blocks = 1u << (log2B - lsc);
+  const size_t lds = slab_lds_bytes(sizeof(Real) == 4, loops, lsc, P);
+  hipLaunchKernelGGL((fold_slab_kernel<Real, PMAX, NT>), dim3(blocks), dim3(NT), lds, st,
+                     (const T2*)x, (const T2*)fT, (T2*)bins, a_dev, ai_dev, mask, log2B, log2m,
+                     w, loops, tiled, T, P, lsc, (T2*)yc, offs, Rl, log2sigma);
+  return hipGetLastError();
+}
+
+template <typename Real, int NT>
+static hipError_t slab_dispatch_nt(int P, const void* x, const void* fT, void* bins,
+                                   const int* a_dev, const int* ai_dev, uint32_t mask, int log2B,
+                                   int log2m, int w, int loops, int tiled, int T, int lsc,
+                                   void* yc, const int* offs, int Rl, int log2sigma,
+                                   hipStream_t st) {
+  if (P <= 16)
+    return slab_launch<Real, 16, NT>(x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w, loops,
+                                     tiled, T, P, lsc, yc, offs, Rl, log2sigma, st);
+  if (P <= 24)
+    return slab_launch<Real, 24, NT>(x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w, loops,
+                                     tiled, T, P, lsc, yc, offs, Rl, log2sigma, st);
+  return slab_launch<Real, 32, NT>(x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w, loops,
+                                   tiled, T, P, lsc, yc, offs, Rl, log2sigma, st);
+}
+
+// 512 threads per workgroup: two chunks of loads in flight per workgroup at 4 loads per thread
+template <typename Real>
+static hipError_t slab_dispatch(int P, const void* x, const void* fT, void* bins,
+                                const int* a_dev, const int* ai_dev, uint32_t mask, int log2B,
+                                int log2m, int w, int loops, int tiled, int T, int lsc, void* yc,
+                                const int* offs, int Rl, int log2sigma, hipStream_t st) {
+  return slab_dispatch_nt<Real, 512>(P, x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w,
+                                     loops, tiled, T, lsc, yc, offs, Rl, log2sigma, st);
+}
+
+int fold_slab_applies(int is_fp32, int n, int B, int w, int loops) {
+  int log2B = 0, log2n = 0;
+  while ((1 << log2B) < B) ++log2B;
+  while ((1LL << log2n) < n) ++log2n;
+  return slab_log2sc(is_fp32, log2B, log2n - log2B, loops, fold_rows_pitch(B, w)) >= 0;
+}
+
+hipError_t filter_rows_launch(int is_fp32, const void* f, void* fT, int B, int w, int tiled,
+                              hipStream_t st) {
+  const int T = w / B, P = fold_rows_pitch(B, w);
+  const long long total = (long long)B * P;
+  const unsigned blocks = (unsigned)((total + 255) / 256);
+  if (is_fp32)
+    hipLaunchKernelGGL(filter_rows_kernel<float2>, dim3(blocks), dim3(256), 0, st,
+                       (const float2*)f, (float2*)fT, B, w, T, P, tiled);
+  else
+    hipLaunchKernelGGL(filter_rows_kernel<double2>, dim3(blocks), dim3(256), 0, st,
+                       (const double2*)f, (double2*)fT, B, w, T, P, tiled);
+  return hipGetLastError();
+}
+
+hipError_t fold_columns_launch(int is_fp32, const void* x, const void* fT, void* bins,
+                               const int* a_dev, const int* ai_dev, int n, int B, int w,
+                               int loops, int tiled, int remap, void* yc, const int* offs,
+                               int Rl, int W, hipStream_t st) {
+  int log2B = 0, log2n = 0, log2W = 0;
+  while ((1 << log2B) < B) ++log2B;
+  while ((1LL << log2n) < n) ++log2n;
+  while ((1 << log2W) < W) ++log2W;
+  const int log2m = log2n - log2B, log2sigma = log2n - log2W;
+  if (!yc || log2sigma > log2B) Rl = 0;   // no comb columns to copy (sigma must divide B)
+  const int T = w / B, P = fold_rows_pitch(B, w);
+  const uint32_t mask = (uint32_t)n - 1u;
+  const int lsc = remap >= 0 ? slab_log2sc(is_fp32, log2B, log2m, loops, P) : -1;
+  if (lsc >= 0)
+    return is_fp32 ? slab_dispatch<float>(P, x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w,
+                                          loops, tiled, T, lsc, yc, offs, Rl, log2sigma, st)
+                   : slab_dispatch<double>(P, x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w,
+                                           loops, tiled, T, lsc, yc, offs, Rl, log2sigma, st);
+  // columns per block: one 128-byte line of x (8 complex128 / 16 complex64), fewer if B is
+  // smaller
+  int log2lw = is_fp32 ? 4 : 3;
+  if (log2lw > log2B) log2lw = log2B;
+  const long long total = (long long)loops * B;
+  const unsigned blocks = (unsigned)((total + FC_NT - 1) / FC_NT);
+  if (remap < 0) remap = 1;
+  if (is_fp32)
+    hipLaunchKernelGGL(fold_columns_kernel<float>, dim3(blocks), dim3(FC_NT), 0, st,
+                       (const float2*)x, (const float2*)fT, (float2*)bins, a_dev, ai_dev, mask,
+                       log2B, w, loops, tiled, T, P, log2lw, remap, (float2*)yc, offs, Rl,
+                       log2sigma, log2m);
+  else
+    hipLaunchKernelGGL(fold_columns_kernel<double>, dim3(blocks), dim3(FC_NT), 0, st,
+                       (const double2*)x, (const double2*)fT, (double2*)bins, a_dev, ai_dev,
+                       mask, log2B, w, loops, tiled, T, P, log2lw, remap, (double2*)yc, offs,
+                       Rl, log2sigma, log2m);
+  return hipGetLastError();
 }
 
 // ============================================================================================

@@ -214,7 +214,7 @@ extern "C" sfft_status sfft_params_validate(const sfft_params* p) {
   if (p->fold_mode != SFFT_FOLD_REFERENCE && p->fold_mode != SFFT_FOLD_ALL_TAPS)
     return SFFT_ERR_INVALID;
   if (p->hit_capacity < 0) return SFFT_ERR_INVALID;
-  if (p->front_end < SFFT_FRONT_AUTO || p->front_end > SFFT_FRONT_STREAM) return SFFT_ERR_INVALID;
+  if (p->front_end < SFFT_FRONT_AUTO || p->front_end > SFFT_FRONT_COLUMNS) return SFFT_ERR_INVALID;
   if (p->W_RLFC < p->n / p->B) return SFFT_ERR_INVALID;   // the comb block must span a window
   return SFFT_OK;
 }
@@ -250,10 +250,21 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   };
   std::vector<Item> items;
   items.push_back({(void**)&P->d_a, (size_t)(2 * L + Rl) * 4});   // a | ai | offsets
-  P->stream = prm->front_end == SFFT_FRONT_STREAM;   // AUTO = GATHER
+  P->stream = prm->front_end == SFFT_FRONT_STREAM;
+  // AUTO: the column front end where its LDS-staged slab fold applies to an FP32 plan (it
+  // streams x once, 1 GiB at n = 2^27, against one random line per tap for the gathers); FP64
+  // plans gather (measured at n = 2^27, k = 16000: streaming the 2 GiB of complex128 costs more
+  // than gathering the 20 M taps)
+  P->columns = prm->front_end == SFFT_FRONT_COLUMNS ||
+               (prm->front_end == SFFT_FRONT_AUTO && P->fp32 &&
+                fold_slab_applies(1, prm->n, prm->B, prm->w, L));
   if (P->stream) {
     items.push_back({&P->taps, (size_t)L * prm->w * e});
     items.push_back({&P->comb_y, (size_t)Rl * W * e});
+  }
+  if (P->columns) {
+    items.push_back({&P->frows, B * (size_t)fold_rows_pitch((int)B, prm->w) * e});
+    items.push_back({&P->comb_y, (size_t)Rl * W * e});   // comb columns copied by the fold
   }
   items.push_back({&P->comb0, (size_t)Rl * W * e});
   items.push_back({&P->comb1, (size_t)Rl * W * e});
@@ -356,6 +367,11 @@ extern "C" sfft_status sfft_plan_set_filter(sfft_plan* P, const void* d_filter_t
   drop_plan_graph(P);
   if (hipMemcpy(P->win, h_filter_win, (size_t)P->m * 16, hipMemcpyHostToDevice) != hipSuccess)
     return SFFT_ERR_HIP;
+  if (P->columns) {   // the column fold's bucket-major copy of the taps
+    const int tiled = (P->p.fold_mode == SFFT_FOLD_REFERENCE) && (P->p.n < (1 << 27));
+    HIPCK(filter_rows_launch(P->fp32, d_filter_time, P->frows, P->p.B, P->p.w, tiled, P->st));
+    HIPCK(hipStreamSynchronize(P->st));
+  }
   P->d_ft = d_filter_time;
   P->has_filter = 1;
   return SFFT_OK;
@@ -453,6 +469,14 @@ static FftLaunch comb_fft(const sfft_plan* P, const void* d_x) {
     c.in = P->comb_y;
     c.in_bstride = W;
     c.load = kLoadPlain;
+  } else if (P->columns) {
+    // the fold copied the comb samples column-major: sample j = row * (B/sigma) + lo at
+    // comb_y[lo * m + row]
+    c.in = P->comb_y;
+    c.in_bstride = W;
+    c.load = kLoadCombRows;
+    c.comb_log2nlo = P->log2W - (P->log2n - P->log2B);
+    c.comb_log2m = P->log2n - P->log2B;
   } else {
     c.load = kLoadComb;
     c.comb_x = d_x;
@@ -487,6 +511,10 @@ sfft_status run_fold(sfft_plan* P, const void* d_x, hipStream_t st) {
   if (P->stream)
     HIPCK(fold_taps_launch(P->fp32, P->taps, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, p.loops,
                            tiled, st));
+  else if (P->columns)
+    HIPCK(fold_columns_launch(P->fp32, d_x, P->frows, P->bins_t, P->d_a, P->d_ai, p.n, p.B, p.w,
+                              p.loops, tiled, 1, P->comb_y, P->d_offs, p.RLFC_loops, p.W_RLFC,
+                              st));
   else
     HIPCK(perm_filter_launch(P->fp32, d_x, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, p.loops,
                              tiled, st));
@@ -537,6 +565,39 @@ sfft_status run_back(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t
   return v != SFFT_OK ? v : run_locate(P, h, d_nodes, st, marks);
 }
 
+// Column front end: the fold also copies the comb samples, so it runs first; then the comb
+// chain (comb FFT from the copied samples, cutoff, bitmaps) on the plan stream and the bucket
+// chain (bucket FFT, cutoff) on the second stream, joined before the reverse hash.  Profiling
+// runs them serially; its "streaming pass" interval is then the fold (with the comb copy) and
+// its "fold" interval is empty.
+static sfft_status run_all_columns(sfft_plan* P, const void* d_x, const HashArgs& h,
+                                   Node* d_nodes) {
+  sfft_status v;
+  const bool prof = P->profile != 0;
+  auto mark = [&](int i) -> hipError_t {
+    return prof ? hipEventRecord(P->ev[i], P->st) : hipSuccess;
+  };
+  HIPCK(mark(0));
+  HIPCK(hash_clear_launch(h, P->st));
+  HIPCK(mark(1));
+  if ((v = run_fold(P, d_x, P->st)) != SFFT_OK) return v;
+  HIPCK(mark(2));
+  if (prof) {
+    HIPCK(fft_run(comb_fft(P, d_x), P->st));
+    if ((v = run_cut(P, h, P->st, true)) != SFFT_OK) return v;
+    if ((v = run_bucket(P, P->st, true)) != SFFT_OK) return v;
+    return run_locate(P, h, d_nodes, P->st, true);
+  }
+  HIPCK(hipEventRecord(P->ev_fork, P->st));
+  HIPCK(hipStreamWaitEvent(P->st2, P->ev_fork, 0));
+  if ((v = run_bucket(P, P->st2, false)) != SFFT_OK) return v;
+  HIPCK(hipEventRecord(P->ev_join, P->st2));
+  HIPCK(fft_run(comb_fft(P, d_x), P->st));
+  if ((v = run_cut(P, h, P->st, false)) != SFFT_OK) return v;
+  HIPCK(hipStreamWaitEvent(P->st, P->ev_join, 0));
+  return run_locate(P, h, d_nodes, P->st, false);
+}
+
 // One transform as two concurrent chains joined before the reverse hash: the comb chain
 // (clear, comb FFT, cutoff, bitmaps) on the plan stream and the bucket chain (permute+filter,
 // bucket FFT, cutoff) on a second stream, so the latency-bound kernels of one chain fill the
@@ -544,6 +605,7 @@ sfft_status run_back(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t
 // fold) runs everything serially on the plan stream.
 static sfft_status run_all(sfft_plan* P, const void* d_x, const HashArgs& h, Node* d_nodes) {
   sfft_status v;
+  if (P->columns) return run_all_columns(P, d_x, h, d_nodes);
   if (P->profile || P->stream) {   // (the streaming pass feeds the fold: one chain)
     if ((v = run_front(P, d_x, h, P->st, true)) != SFFT_OK) return v;
     if ((v = run_fold(P, d_x, P->st)) != SFFT_OK) return v;

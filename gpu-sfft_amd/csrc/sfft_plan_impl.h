@@ -27,6 +27,8 @@ struct sfft_plan {
   int stream = 0;               // streaming front end (else gathers)
   void* taps = nullptr;         // loops x w tap values (stream)
   void* comb_y = nullptr;       // RLFC_loops x W comb samples (stream)
+  int columns = 0;              // column-ordered fold (SFFT_FRONT_COLUMNS)
+  void* frows = nullptr;        // bucket-major filter taps of the column fold
   void* bins_t = nullptr;
   void* bins_scr = nullptr;
   void* bins_f = nullptr;

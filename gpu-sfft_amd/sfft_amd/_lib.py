@@ -6,9 +6,19 @@ missing or fails to load, importing this module raises, naming the file and the 
 import ctypes as C
 import os
 
+try:
+    # torch ships its own HIP runtime (torch/lib/libamdhip64.so, soname libamdhip64.so.7).
+    # Loaded first, it is the runtime libsfft.so binds to; if libsfft.so came first it would
+    # bring in /opt/rocm's copy and torch would load a second runtime beside it, which then
+    # sees no device.  So torch (when installed) is imported before the library is opened.
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 _PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.dirname(_PKG_DIR)
-LIB_PATH = os.path.join(LIB_DIR, "libsfft.so")
+# SFFT_LIBRARY: another build of the same sources (an instrumented or experimental one)
+LIB_PATH = os.environ.get("SFFT_LIBRARY") or os.path.join(LIB_DIR, "libsfft.so")
 CSRC_DIR = os.path.join(LIB_DIR, "csrc")
 
 
@@ -51,11 +61,13 @@ SFFT_FOLD_ALL_TAPS = 1
 SFFT_FRONT_AUTO = 0
 SFFT_FRONT_GATHER = 1
 SFFT_FRONT_STREAM = 2
+SFFT_FRONT_COLUMNS = 3
 
 # every symbol include/sfft.h declares (functions and data)
 EXPORTED_FUNCTIONS = [
     "outer_loop", "restr_loc_freq_coeff", "samples_indices", "perm_filter_cutoff",
-    "reverse_hash", "estimate_values", "timesmod", "make_multiple_t", "cudafft_experiment",
+    "reverse_hash", "estimate_values", "timesmod", "make_multiple_t",
+    "cudafft_experiment",
     "run_experiment", "cuShift", "sfft_srandom", "sfft_random",
     "sfft_status_string", "sfft_params_validate", "sfft_plan_create", "sfft_plan_destroy",
     "sfft_plan_workspace_bytes", "sfft_plan_set_filter", "sfft_execute_async", "sfft_execute",

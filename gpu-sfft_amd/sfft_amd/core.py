@@ -18,7 +18,8 @@ assert NODE_DTYPE.itemsize == C.sizeof(L.Node) == 24
 
 FOLD_REFERENCE = L.SFFT_FOLD_REFERENCE
 FOLD_ALL_TAPS = L.SFFT_FOLD_ALL_TAPS
-FRONT = {"auto": L.SFFT_FRONT_AUTO, "gather": L.SFFT_FRONT_GATHER, "stream": L.SFFT_FRONT_STREAM}
+FRONT = {"auto": L.SFFT_FRONT_AUTO, "gather": L.SFFT_FRONT_GATHER, "stream": L.SFFT_FRONT_STREAM,
+         "columns": L.SFFT_FRONT_COLUMNS}
 
 
 class SfftError(RuntimeError):

@@ -148,14 +148,23 @@ typedef struct {
   int front_end;      /* sfft_front_end: how the taps and comb samples are read from x */
 } sfft_params;
 
-/* Both front ends produce bit-identical results.  GATHER reads x[(i*ai) mod n] per tap and
- * x[off + j*sigma] per comb sample (random HBM reads).  STREAM reads x once in address order
- * and scatters each used element to a tap-ordered staging array (loops*w entries) and the
- * comb arrays, which the fold then reads contiguously.  The scatter is as random as the
- * gathers it replaces (one partial cache line per used element), so on MI355X at n = 2^27 it
- * is slower (FP64) or even (FP32); AUTO therefore picks GATHER.  STREAM stays available for
- * signals whose used fraction loops*w/n approaches one. */
-typedef enum { SFFT_FRONT_AUTO = 0, SFFT_FRONT_GATHER = 1, SFFT_FRONT_STREAM = 2 } sfft_front_end;
+/* Every front end produces bit-identical results.  GATHER reads x[(i*ai) mod n] per tap in
+ * bucket order and x[off + j*sigma] per comb sample (random HBM reads).  STREAM reads x once
+ * in address order and scatters each used element to a tap-ordered staging array (loops*w
+ * entries) and the comb arrays, which the fold then reads contiguously; the scatter is as
+ * random as the gathers it replaces (one partial cache line per used element).  COLUMNS folds
+ * in column order: all taps of bucket r lie in column (r*ai) mod B of x viewed as n/B rows of
+ * B columns, so the fold walks the columns in blocks of one 128-byte line width for every loop
+ * at once and the taps of different loops that fall in the same line share its fetch; the
+ * filter taps are read from a bucket-major copy built when the filter is set (B*(w/B+1)
+ * entries in the plan workspace).  AUTO picks COLUMNS for FP32 plans (where the fold streams
+ * the 256-byte-wide column slabs through LDS) and GATHER for FP64 plans. */
+typedef enum {
+  SFFT_FRONT_AUTO = 0,
+  SFFT_FRONT_GATHER = 1,
+  SFFT_FRONT_STREAM = 2,
+  SFFT_FRONT_COLUMNS = 3
+} sfft_front_end;
 
 typedef struct sfft_plan sfft_plan;
 

@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# SFFT_ORACLE_LIBRARY: another build of the same source (the sanitizer run's instrumented one)
+_LIB_PATH = os.environ.get("SFFT_ORACLE_LIBRARY") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 

@@ -90,7 +90,8 @@ def test_validation_mirrors_reference_asserts(S):
         assert S.validate(q) == 1, q
     assert S.validate(p, fold_mode=7) == 1
     assert S.validate(p, hit_capacity=-1) == 1
-    assert S.validate(p, front_end=3) == 1 and S.validate(p, front_end="stream") == 0
+    assert S.validate(p, front_end=4) == 1 and S.validate(p, front_end="stream") == 0
+    assert S.validate(p, front_end="columns") == 0
 
 
 def test_plan_without_device_reports_status(S):

@@ -106,13 +106,16 @@ def test_transform_all_taps_bitexact(S, cfg):
 
 @pytest.mark.parametrize("cfg", SMALL_CONFIGS + [N20_K1000])
 @pytest.mark.parametrize("fold_mode", [O.FOLD_REFERENCE, O.FOLD_ALL_TAPS])
-def test_transform_stream_front_end_bitexact(S, cfg, fold_mode):
-    """Streaming front end (one pass over x fills the tap rows and the comb samples, then the
-    fold reads the rows): same arithmetic as the gather path, so the same bits."""
+@pytest.mark.parametrize("front_end", ["gather", "stream", "columns"])
+def test_transform_front_ends_bitexact(S, cfg, fold_mode, front_end):
+    """Every front end: gathers in bucket order; the streaming pass (one pass over x fills the
+    tap rows and the comb samples, then the fold reads the rows); the column-ordered fold
+    (buckets walked by column of x, filter taps from the bucket-major copy).  Same arithmetic
+    in the same order, so the same bits."""
     if fold_mode == O.FOLD_ALL_TAPS and cfg == N20_K1000:
         pytest.skip("covered by the small configurations")
     prm, plan, ref, nodes, cnt, _ = _run_both(S, cfg, seed=cfg[0] + 3, fold_mode=fold_mode,
-                                              front_end="stream")
+                                              front_end=front_end)
     _assert_transform_equal(prm, plan, ref, nodes, cnt)
 
 
@@ -314,7 +317,7 @@ def test_reference_outer_loop_entry_point(S):
 
 # ---- the FP32 path: complex64 storage, FP32 FFTs and Kahan fold, FP64 estimation ---------
 @pytest.mark.parametrize("cfg", [SMALL_CONFIGS[1], SMALL_CONFIGS[3], N20_K1000])
-@pytest.mark.parametrize("front_end", ["gather", "stream"])
+@pytest.mark.parametrize("front_end", ["gather", "stream", "columns"])
 def test_transform_fp32_bitexact(S, cfg, front_end):
     _fp32_transform_check(S, cfg, front_end)
 

@@ -317,3 +317,20 @@ def test_estimate_values_exact_medians():
         node = O.estimate_values(np.array([I], dtype=np.int32), b, loops, n, ai, B, win)[0]
         assert node["key"] == I
         assert node["re"] == np.sort(vals.real)[med] and node["im"] == np.sort(vals.imag)[med]
+
+
+@pytest.mark.parametrize("log2n,log2B", [(12, 5), (20, 13), (27, 17)])
+def test_bucket_taps_share_one_column(log2n, log2B):
+    """The fact the column-ordered fold rests on: every tap id = t*B + r of bucket r gathers
+    x[(id*ai) mod n], whose column (id*ai) mod B is (r*ai) mod B for every t; column c holds
+    bucket (c*a) mod B of the loop, a = ai^-1 mod n."""
+    rng = np.random.default_rng(log2n)
+    n, B = 1 << log2n, 1 << log2B
+    for _ in range(4):
+        a = int(rng.integers(0, n // 2)) * 2 + 1
+        ai = pow(a, -1, n)
+        r = rng.integers(0, B, 64).astype(np.int64)
+        t = rng.integers(0, 40, 64).astype(np.int64)
+        j = ((t * B + r) * ai) % n
+        assert np.array_equal(j % B, (r * ai) % B)
+        assert np.array_equal(((j % B) * a) % B, r)
