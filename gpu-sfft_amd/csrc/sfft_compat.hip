@@ -268,6 +268,61 @@ extern "C" int samples_indices(complex_t* d_samp, int B, int samp_size, int* d_o
   return rc;
 }
 
+// samples_indices without allocation: the |X|^2 keys and the select workspace live in one
+// file-scope device array (each device holds its own copy for the life of the process).  Host
+// code reaches it only through the address hipGetSymbolAddress returns on that device, looked
+// up once per device and kept.
+static constexpr int kNoAllocMaxN = SFFT_NOALLOC_MAX_SAMPLES;
+static constexpr size_t kNoAllocBytes = (size_t)12 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+static void* scratch_on_current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+extern "C" int samples_indices_noalloc(complex_t* d_samp, int B, int samp_size, int* d_output,
+                                       double* CTime) {
+  const double t0 = wall_now();
+  if (B < 1 || B >= samp_size) return -1;   // as samples_indices
+  if (samp_size > kNoAllocMaxN) {
+    fprintf(stderr, "samples_indices_noalloc: samp_size %d exceeds the scratch maximum %d\n",
+            samp_size, kNoAllocMaxN);
+    return -2;
+  }
+  const size_t keys_bytes = ((size_t)samp_size * 8 + 255) / 256 * 256;
+  const size_t work_bytes = select_work_bytes(1, samp_size);
+  if (keys_bytes + work_bytes > kNoAllocBytes) return -2;   // cannot happen below the maximum
+  char* base = (char*)scratch_on_current_device();
+  if (!base) return -1;
+  unsigned long long* keys = (unsigned long long*)base;
+  void* work = base + keys_bytes;
+  // the select state must start at zero; the previous call left its own contents behind
+  if (fill_u32(work, 0u, (work_bytes + 3) / 4, nullptr) != hipSuccess) return -1;
+  hipLaunchKernelGGL(keys_c128_kernel, dim3((samp_size + 255) / 256), dim3(256), 0, nullptr,
+                     (const double2*)d_samp, keys, samp_size);
+  if (hipGetLastError() != hipSuccess) return -1;
+  SelectWork sw;
+  select_work_bind(&sw, work, 1, samp_size);
+  const hipError_t e = select_topk(0, keys, samp_size, samp_size, B, 1, d_output, 1, sw, nullptr);
+  const hipError_t s = hipStreamSynchronize(nullptr);
+  if (CTime) *CTime += wall_now() - t0;
+  return (e == hipSuccess && s == hipSuccess) ? 0 : -1;
+}
+
 extern "C" int restr_loc_freq_coeff(complex_t* d_x, int n, int B_thresh, int W_RLFC, int* d_J2,
                                     double* RLFC_Time) {   // innerLoop.cu:112-177
   const double t0 = wall_now();

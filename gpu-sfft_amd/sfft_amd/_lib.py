@@ -66,7 +66,7 @@ SFFT_FRONT_COLUMNS = 3
 # every symbol include/sfft.h declares (functions and data)
 EXPORTED_FUNCTIONS = [
     "outer_loop", "restr_loc_freq_coeff", "samples_indices", "perm_filter_cutoff",
-    "reverse_hash", "estimate_values", "timesmod", "make_multiple_t",
+    "reverse_hash", "estimate_values", "timesmod", "samples_indices_noalloc", "make_multiple_t",
     "cudafft_experiment",
     "run_experiment", "cuShift", "sfft_srandom", "sfft_random",
     "sfft_status_string", "sfft_params_validate", "sfft_plan_create", "sfft_plan_destroy",
@@ -204,6 +204,8 @@ def _declare(L):
     L.restr_loc_freq_coeff.restype = _i
     L.samples_indices.argtypes = [_p, _i, _i, _p, C.POINTER(_d)]
     L.samples_indices.restype = _i
+    L.samples_indices_noalloc.argtypes = [_p, _i, _i, _p, C.POINTER(_d)]
+    L.samples_indices_noalloc.restype = _i
     L.perm_filter_cutoff.argtypes = [_p, _i, _p, _i, _i, _i, _i, _i, _p, _p, C.POINTER(_d),
                                      C.POINTER(_d)]
     L.perm_filter_cutoff.restype = _i

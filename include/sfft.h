@@ -89,6 +89,18 @@ Node *estimate_values(int *d_I, int *I_F, complex_t *d_bins_f, int loops, int n,
 /* innerLoop.h:50 */
 int timesmod(const int x, const int a, const int n);
 
+/* No counterpart in the reference: samples_indices (above) for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, result and synchronisation as samples_indices, but the |X|^2
+ * keys and the select workspace live in a file-scope device array of 12 MiB that each device
+ * holds until the process ends; no hipMalloc/hipFree.  samp_size may be at most
+ * SFFT_NOALLOC_MAX_SAMPLES: larger sizes return -2 (and print why to stderr), other argument
+ * errors -1 as samples_indices.  Every call on one device shares that device's array, so
+ * calls that overlap on one GPU (from several host threads) race: serialise them. */
+#define SFFT_NOALLOC_MAX_SAMPLES 262144
+int samples_indices_noalloc(complex_t *d_samp, int B, int samp_size, int *d_output,
+                            double *CTime);
+
 /* outer_loop draws its permutations a[l] with glibc's random() generator in the reference's
  * order (outerLoop.cu:129-134) -- on a library-private state, because the HIP runtime calls
  * random() from its own threads and would make the process-wide sequence timing-dependent.

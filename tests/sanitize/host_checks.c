@@ -145,6 +145,10 @@ static void check_no_device_paths(void) {
   sfft_batch* b = (sfft_batch*)0x1;
   CHECK(sfft_batch_create(&p, 1, SFFT_FP32, NULL, &b) == SFFT_ERR_INVALID && b == NULL);
   CHECK(sfft_batch_create(NULL, 1, SFFT_FP32, NULL, &b) == SFFT_ERR_INVALID);
+  double t = 0;
+  /* size limit and argument errors are decided before any device call */
+  CHECK(samples_indices_noalloc(NULL, 10, SFFT_NOALLOC_MAX_SAMPLES + 1, NULL, &t) == -2);
+  CHECK(samples_indices_noalloc(NULL, 0, 100, NULL, &t) == -1);
 }
 
 int main(void) {
