@@ -25,6 +25,12 @@ struct CT<float> {
   using K = unsigned int;
 };
 
+// complex element as a native vector (arrays of HIP's double2/float2 wrappers end up in scratch)
+template <typename Real>
+struct NV {
+  typedef Real type __attribute__((ext_vector_type(2)));
+};
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 w) {
   return make_double2(fma(a.x, w.x, -(a.y * w.y)), fma(a.x, w.y, a.y * w.x));
 }
@@ -229,7 +235,9 @@ __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol,
 }
 
 template <typename Real, int LOAD, int STORE>
-__global__ __launch_bounds__((Tile<Real, LOAD>::nt)) void fft_pass_kernel(PassArgs A) {
+// 8 waves per SIMD for the 1024-thread tiles: two workgroups per CU
+__global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void fft_pass_kernel(
+    PassArgs A) {
   using T2 = typename CT<Real>::T2;
   using KT = typename CT<Real>::K;
   constexpr int CAP = Tile<Real, LOAD>::cap, EPT = Tile<Real, LOAD>::ept;
@@ -271,12 +279,25 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt)) void fft_pass_kernel(PassAr
   const int ncol = 1 << log2ncol;
   const int TE = ncol << r;
 
-  // ---- load (+ outer twiddle) into LDS, column c at lds[c*RP + v]
+  // ---- load (+ outer twiddle) into LDS, column c at lds[c*RP + v].  Every element load of
+  // the thread (and every twiddle load) is issued before any is used: in one basic block,
+  // with out-of-tile elements clamped to a valid address, so the loads are in flight together
+  // (guarding each load with its own branch made the compiler wait for each one in turn).
+  using V2 = typename NV<Real>::type;
   const T2* __restrict__ in = (const T2*)A.in + b * A.in_bstride;
+  const T2* __restrict__ cx =
+      (LOAD == kLoadComb) ? (const T2*)A.comb_x + A.comb_offs[b] : (const T2*)nullptr;
+  // loads in flight per batch: the plain tiles take batches of 8 bytes x 4 (FP32) or
+  // 16 bytes x 2 (FP64), which keeps them within 64 registers (two 1024-thread workgroups
+  // per CU); the comb-gathering pass issues all of its loads together
+  constexpr int LB = (LOAD == kLoadComb) ? EPT : (sizeof(Real) == 8 ? 2 : 4);
+  const bool split = log2Lcur > kMaxPassLog;
 #pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const int e = tid + k * NT;
-    if (e < TE) {
+  for (int kb = 0; kb < EPT; kb += LB) {
+    V2 vv[LB];
+#pragma unroll
+    for (int k = 0; k < LB; ++k) {
+      const int e = min(tid + (kb + k) * NT, TE - 1);
       const int c = e & (ncol - 1);
       const int v = e >> log2ncol;
       long long g;
@@ -284,15 +305,78 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt)) void fft_pass_kernel(PassAr
         g = q0 + c + nq * v;
       else
         g = (q + nq * v) * Lprev + t0 + c;
-      T2 val;
+      const T2* src;
       if (LOAD == kLoadComb)
-        val = ((const T2*)A.comb_x)[(long long)A.comb_offs[b] + g * (long long)A.comb_sigma];
+        src = cx + g * (long long)A.comb_sigma;
       else if (LOAD == kLoadCombRows)
-        val = in[((g & ((1LL << A.log2nlo) - 1)) << A.log2m) | (g >> A.log2nlo)];
+        src = in + (((g & ((1LL << A.log2nlo) - 1)) << A.log2m) | (g >> A.log2nlo));
       else
-        val = in[g];
-      if (!first && v > 0) val = cmul(val, outer_tw<T2>(A, (uint32_t)((t0 + c) * v), log2Lcur));
-      lds[c * RP + LPAD(v)] = val;
+        src = in + g;
+      vv[k] = *(const V2*)src;
+    }
+    if (!first) {
+      // outer twiddles W_Lcur^((t0+c)*v): a direct table entry, or the product of a high and
+      // a low table entry (every table load of the batch issued first, as for the data)
+      V2 tw[LB], tl[LB];
+      if (split) {
+        const V2* __restrict__ thi = (const V2*)A.thi;
+        const V2* __restrict__ tlo = (const V2*)A.tlo;
+        const uint32_t lmask = (1u << A.s) - 1u;
+#pragma unroll
+        for (int k = 0; k < LB; ++k) {
+          const int e = min(tid + (kb + k) * NT, TE - 1);
+          const uint32_t ex = (uint32_t)((t0 + (e & (ncol - 1))) * (e >> log2ncol));
+          tw[k] = thi[ex >> A.s];
+          tl[k] = tlo[ex & lmask];
+        }
+#pragma unroll
+        for (int k = 0; k < LB; ++k) {
+          T2 hi, lo;
+          hi.x = tw[k].x;
+          hi.y = tw[k].y;
+          lo.x = tl[k].x;
+          lo.y = tl[k].y;
+          const T2 w = cmul(hi, lo);
+          tw[k].x = w.x;
+          tw[k].y = w.y;
+        }
+      } else {
+        const V2* __restrict__ tdir = (const V2*)A.tdir;
+#pragma unroll
+        for (int k = 0; k < LB; ++k) {
+          const int e = min(tid + (kb + k) * NT, TE - 1);
+          const uint32_t ex = (uint32_t)((t0 + (e & (ncol - 1))) * (e >> log2ncol));
+          tw[k] = tdir[ex];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < LB; ++k) {
+        const int e = tid + (kb + k) * NT;
+        if (e < TE) {
+          const int c = e & (ncol - 1);
+          const int v = e >> log2ncol;
+          T2 val, w;
+          val.x = vv[k].x;
+          val.y = vv[k].y;
+          w.x = tw[k].x;
+          w.y = tw[k].y;
+          if (v > 0) val = cmul(val, w);
+          lds[c * RP + LPAD(v)] = val;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < LB; ++k) {
+        const int e = tid + (kb + k) * NT;
+        if (e < TE) {
+          const int c = e & (ncol - 1);
+          const int v = e >> log2ncol;
+          T2 val;
+          val.x = vv[k].x;
+          val.y = vv[k].y;
+          lds[c * RP + LPAD(v)] = val;
+        }
+      }
     }
   }
   __syncthreads();
@@ -803,11 +887,6 @@ __global__ __launch_bounds__(FC_NT) void fold_columns_kernel(
 // last chunk it adds its slots up in the canonical order.  The comb columns of the slab are
 // copied out of each chunk.  Same per-bucket terms and order as pf_body: the same bits.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-// complex element as a native vector (arrays of HIP's double2/float2 wrappers end up in scratch)
-template <typename Real>
-struct NV {
-  typedef Real type __attribute__((ext_vector_type(2)));
-};
 constexpr int SL_ROWS = 128;  // rows per chunk
 constexpr int SL_PITCH = 17;  // 16-byte units per chunk row: 256 bytes + 16 of padding (banks)
 constexpr int SL_U = 4;       // 16-byte loads per thread for one chunk (512 threads)
@@ -844,13 +923,13 @@ __global__ __launch_bounds__(NT) void fold_slab_kernel(
   const uint32_t smask = (1u << log2sigma) - 1u;
   const size_t W = (size_t)B << log2m >> log2sigma;
   u32x4 nx[SL_U];
+  // the loads of a chunk in one basic block (clamped addresses): in flight together
   auto load = [&](int ch) {
 #pragma unroll
     for (int u = 0; u < SL_U; ++u) {
-      const int i = u * NT + tid;
-      if (u * NT < units && i < units)
-        nx[u] = __builtin_nontemporal_load(
-            &x16[((size_t)ch * rows + (size_t)(i >> 4)) * row16 + col16 + (size_t)(i & 15)]);
+      const int i = min(u * NT + tid, units - 1);
+      nx[u] = __builtin_nontemporal_load(
+          &x16[((size_t)ch * rows + (size_t)(i >> 4)) * row16 + col16 + (size_t)(i & 15)]);
     }
   };
   auto store = [&]() {
