@@ -8,6 +8,8 @@
 // Every floating-point operation that must match the CPU oracle bit for bit is written out
 // explicitly (fma where fused, plain ops elsewhere); the library is built with
 // -ffp-contract=off so the compiler adds no fusion of its own.
+#include <algorithm>
+
 #include "sfft_internal.h"
 
 namespace sfft {
@@ -92,7 +94,7 @@ constexpr int FFT_QMAX = 16;                 // columns per tile
 // or 256 (FP32: eight), so more gathers are in flight while other workgroups compute
 // (measured at n = 2^27: 256 threads cost FP64 10 us, 1024 cost it 65 us; FP32 gains 6 us).
 template <typename Real, int LOAD = 0>
-struct Tile {   // LOAD: kLoadPlain, kLoadComb (gathers from x) or kLoadCombRows (plain tile)
+struct Tile {   // LOAD: kLoadComb (gathers from x); kLoadPlain, kLoadCombRows, kLoadPerm (plain)
   static constexpr int cap = LOAD == 1 ? 2048 : (sizeof(Real) == 8 ? 4096 : 8192);
   // threads per workgroup
   static constexpr int nt = LOAD == 1 ? (sizeof(Real) == 8 ? 512 : 256) : 1024;
@@ -133,6 +135,7 @@ struct PassArgs {
   int keys_tiled;   // store keys at tile * TE + e (the tile's own contiguous range)
   uint16_t* digits; // optional first digits of the keys (same positions)
   int log2nlo, log2m;   // kLoadCombRows: g at in[(g mod 2^log2nlo) << log2m | g >> log2nlo]
+  const int* perm;      // kLoadPerm: g of batch b at in[g * perm[b] mod 2^log2N]
 };
 
 template <typename T2>
@@ -287,6 +290,7 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
   const T2* __restrict__ in = (const T2*)A.in + b * A.in_bstride;
   const T2* __restrict__ cx =
       (LOAD == kLoadComb) ? (const T2*)A.comb_x + A.comb_offs[b] : (const T2*)nullptr;
+  const uint32_t pm = (LOAD == kLoadPerm) ? (uint32_t)A.perm[b] : 0u;
   // loads in flight per batch: the plain tiles take batches of 8 bytes x 4 (FP32) or
   // 16 bytes x 2 (FP64), which keeps them within 64 registers (two 1024-thread workgroups
   // per CU); the comb-gathering pass issues all of its loads together
@@ -310,6 +314,8 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
         src = cx + g * (long long)A.comb_sigma;
       else if (LOAD == kLoadCombRows)
         src = in + (((g & ((1LL << A.log2nlo) - 1)) << A.log2m) | (g >> A.log2nlo));
+      else if (LOAD == kLoadPerm)
+        src = in + (((uint32_t)g * pm) & ((1u << A.log2N) - 1u));
       else
         src = in + g;
       vv[k] = *(const V2*)src;
@@ -443,6 +449,11 @@ static hipError_t dispatch_pass(const PassArgs& A, int load, int store, int tile
       return launch_pass<Real, kLoadCombRows, kStoreKeys>(A, tiles, batch, st);
     return launch_pass<Real, kLoadCombRows, kStoreBoth>(A, tiles, batch, st);
   }
+  if (load == kLoadPerm) {
+    if (store == kStoreCplx) return launch_pass<Real, kLoadPerm, kStoreCplx>(A, tiles, batch, st);
+    if (store == kStoreKeys) return launch_pass<Real, kLoadPerm, kStoreKeys>(A, tiles, batch, st);
+    return launch_pass<Real, kLoadPerm, kStoreBoth>(A, tiles, batch, st);
+  }
   if (load == kLoadComb) {
     if (store == kStoreCplx) return launch_pass<Real, kLoadComb, kStoreCplx>(A, tiles, batch, st);
     if (store == kStoreKeys) return launch_pass<Real, kLoadComb, kStoreKeys>(A, tiles, batch, st);
@@ -529,6 +540,7 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.comb_sigma = L.comb_sigma;
     A.log2nlo = L.comb_log2nlo;
     A.log2m = L.comb_log2m;
+    A.perm = L.perm;
     A.nbatch = L.batch;
     A.keys_tiled = (last && p > 0 && L.keys_tiled) ? 1 : 0;
     A.digits = last ? L.digits : nullptr;
@@ -876,23 +888,33 @@ __global__ __launch_bounds__(FC_NT) void fold_columns_kernel(
   }
 }
 
-// LDS-staged column fold.  A workgroup owns a slab of SC adjacent columns (SC * e = 256 bytes,
-// two 128-byte lines per row: narrower slabs read at a power-of-two row stride reach only half
-// the bandwidth on MI355X) over all m rows of the view, and streams it through LDS in chunks of
-// SL_ROWS rows (16-byte loads; the next chunk's loads are in flight in registers while the
-// current chunk is used): across the grid the whole signal is read once.  Each (loop, column)
-// thread first loads its bucket's filter row into its slots of an LDS tap array (slot t of
-// thread p at t * pairs + p); as the rows of its taps pass (tap t lies in row
-// ((r*ai + t*B*ai) mod n) / B) it replaces slot t by the tap's product term x * f; after the
-// last chunk it adds its slots up in the canonical order.  The comb columns of the slab are
-// copied out of each chunk.  Same per-bucket terms and order as pf_body: the same bits.
+// LDS-staged column fold.  A slab is SC adjacent columns (SC * e = 256 bytes, two 128-byte
+// lines per row: narrower slabs read at a power-of-two row stride reach only half the bandwidth
+// on MI355X) over all m rows of the view.  A workgroup streams its slabs (s, s + G, s + 2G, ...
+// for a grid of G resident workgroups) through LDS in chunks of SL_ROWS rows with 16-byte loads;
+// the next chunk's loads, also across a slab boundary, are in flight in registers while the
+// current chunk is used, so the stream does not stop between slabs: across the grid the whole
+// signal is read once.  For each slab the workgroup holds the filter rows of the buckets of its
+// (loop, column) pairs in an LDS tap array (slot t of pair p at t * pairs + p; the next slab's
+// rows are fetched while the current slab is finished).  As the rows of its taps pass (tap t
+// lies in row ((r*ai + t*B*ai) mod n) / B), the thread of pair p replaces slot t by the tap's
+// product term x * f; after the slab's last chunk it adds its slots up in the canonical order.
+// The bucket sums are written in column order (bins[l*B + c] for the bucket r = c*a_l mod B of
+// column c: whole segments instead of scattered 8/16-byte stores); the bucket FFT reads them
+// back through that permutation (kLoadPerm).  The comb columns of the slab are copied out of
+// each chunk.  Same per-bucket terms and order as pf_body: the same bits.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int SL_ROWS = 128;  // rows per chunk
 constexpr int SL_PITCH = 17;  // 16-byte units per chunk row: 256 bytes + 16 of padding (banks)
 constexpr int SL_U = 4;       // 16-byte loads per thread for one chunk (512 threads)
+constexpr int SL_TABLE = 128; // LDS ints after the slots: a [0,32), ai [32,64), comb offsets [64,128)
+template <typename Real>
+struct SlabFetch {            // filter-row loads per thread per slab (all in flight together)
+  static constexpr int fb = sizeof(Real) == 8 ? 8 : 16;
+};
 
 template <typename Real, int PMAX, int NT>
-__global__ __launch_bounds__(NT) void fold_slab_kernel(
+__global__ __launch_bounds__(NT, 4) void fold_slab_kernel(
     const typename CT<Real>::T2* __restrict__ x, const typename CT<Real>::T2* __restrict__ fT,
     typename CT<Real>::T2* __restrict__ bins, const int* __restrict__ a_dev,
     const int* __restrict__ ai_dev, uint32_t mask, int log2B, int log2m, int w, int loops,
@@ -902,81 +924,177 @@ __global__ __launch_bounds__(NT) void fold_slab_kernel(
   using V2 = typename NV<Real>::type;
   using AC = typename AccOf<Real>::A;
   constexpr int EPU = 16 / (int)sizeof(T2);             // elements per 16-byte unit
+  constexpr int FB = SlabFetch<Real>::fb;
   extern __shared__ u32x4 dyn16[];
   u32x4* chunk16 = dyn16;                              // SL_ROWS x SL_PITCH units
   const V2* ldsv = (const V2*)dyn16;                   // element (row, k): row*PITCH*EPU + k
-  V2* slotv = (V2*)(dyn16 + SL_ROWS * SL_PITCH);       // PMAX x pairs slots
-  const int tid = threadIdx.x;
-  const uint32_t B = 1u << log2B, m = 1u << log2m, SC = 1u << log2sc;
-  const uint32_t c0 = (uint32_t)blockIdx.x << log2sc;   // whole lines per row: no XCD remap
+  V2* slotv = (V2*)(dyn16 + SL_ROWS * SL_PITCH);       // P x pairs slots
   const int pairs = loops << log2sc;
+  int* atab = (int*)(slotv + (size_t)P * pairs);       // SL_TABLE ints
+  const int tid = threadIdx.x;
+  const uint32_t B = 1u << log2B, SC = 1u << log2sc;
+  const uint32_t nslabs = B >> log2sc;
   const bool active = tid < pairs;
   const uint32_t cc = (uint32_t)tid & (SC - 1u), l = (uint32_t)tid >> log2sc;
-  uint32_t r = 0, ai = 0, base = 0, step = 0;
-  int nmain = 0;
   const int residue = w - T * (int)B;
-  const int rows = (int)(m < (uint32_t)SL_ROWS ? m : (uint32_t)SL_ROWS);
-  const int nchunks = (int)(m / (uint32_t)rows);
-  const int units = rows * 16;               // 16-byte units of x per chunk
+  const int log2rows = log2m < 7 ? log2m : 7;          // SL_ROWS = 2^7
+  const int rows = 1 << log2rows;
+  const int nchunks = 1 << (log2m - log2rows);
+  const int units = rows * 16;                         // 16-byte units of x per chunk
   const u32x4* __restrict__ x16 = (const u32x4*)x;
-  const size_t row16 = ((size_t)B * sizeof(T2)) / 16, col16 = ((size_t)c0 * sizeof(T2)) / 16;
+  const uint32_t row16 = (uint32_t)(((size_t)B * sizeof(T2)) / 16);
   const uint32_t smask = (1u << log2sigma) - 1u;
   const size_t W = (size_t)B << log2m >> log2sigma;
-  u32x4 nx[SL_U];
-  // the loads of a chunk in one basic block (clamped addresses): in flight together
-  auto load = [&](int ch) {
+  // comb columns of a slab per comb loop: SC / sigma when sigma < SC, else at most one
+  const int log2ncs = log2sc > log2sigma ? log2sc - log2sigma : 0;
+  const int ncopy = (Rl << log2ncs) << log2rows;       // comb copy elements per chunk
+  const int total = P * pairs;                         // filter slots (at most FB * NT)
+  const uint32_t magic = P > 1 ? 0xFFFFFFFFu / (uint32_t)P + 1u : 0u;   // k / P = umulhi(k, magic)
+  const V2* __restrict__ fTv = (const V2*)fT;
+
+  // The chunks of the workgroup in order: chunk q is chunk (q mod nchunks) of its slab
+  // q / nchunks (slab blockIdx.x + (q / nchunks) * G).  The next chunk is in flight in
+  // registers (xa) while one is used from LDS.  q past the last chunk is clamped to it.
+  const int log2nch = log2m - log2rows;
+  const uint32_t G = gridDim.x;
+  const int nsl = (int)((nslabs - blockIdx.x + G - 1) / G);   // slabs of this workgroup
+  const int Q = nsl << log2nch;
+  u32x4 xa[SL_U];
+  // the loads of a chunk in one basic block (clamped addresses): in flight together.  Unit
+  // indices in 32 bits (n * e / 16 < 2^32).
+  auto load = [&](u32x4* buf, int q) __attribute__((always_inline)) {
+    const int qq = min(q, Q - 1);
+    const uint32_t c0 = (blockIdx.x + (uint32_t)(qq >> log2nch) * G) << log2sc;
+    const int ch = qq & (nchunks - 1);
+    const uint32_t col16 = (uint32_t)(((size_t)c0 * sizeof(T2)) / 16);
 #pragma unroll
     for (int u = 0; u < SL_U; ++u) {
       const int i = min(u * NT + tid, units - 1);
-      nx[u] = __builtin_nontemporal_load(
-          &x16[((size_t)ch * rows + (size_t)(i >> 4)) * row16 + col16 + (size_t)(i & 15)]);
+      const uint32_t unit =
+          ((uint32_t)(ch * rows) + (uint32_t)(i >> 4)) * row16 + col16 + (uint32_t)(i & 15);
+      buf[u] = __builtin_nontemporal_load(&x16[unit]);
     }
   };
-  auto store = [&]() {
+  // every thread stores every one of its units (a clamped one again, with the same value), so
+  // that all of the chunk's loads are known to be complete here
+  auto store = [&](const u32x4* buf) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < SL_U; ++u) {
-      const int i = u * NT + tid;
-      if (u * NT < units && i < units) chunk16[(i >> 4) * SL_PITCH + (i & 15)] = nx[u];
+      const int i = min(u * NT + tid, units - 1);
+      chunk16[(i >> 4) * SL_PITCH + (i & 15)] = buf[u];
     }
   };
-  load(0);
-  // the filter row of the bucket into the slots while the first chunk is on its way
-  if (active) {
-    ai = (uint32_t)ai_dev[l];
-    r = ((c0 + cc) * (uint32_t)a_dev[l]) & (B - 1u);
-    nmain = (!tiled && (int)r < residue) ? T + 1 : T;
-    base = (r * ai) & mask;
-    step = (B * ai) & mask;
-    const V2* __restrict__ frow = (const V2*)(fT + (size_t)r * (size_t)P);
+  // the filter rows of a slab's pairs: entry k = p * P + t of the fetch (entry t of pair p's
+  // row; neighbouring threads read neighbouring entries of a row) goes to slot t * pairs + p;
+  // any thread any entry (clamped addresses; the loads in flight together)
+  V2 fv[FB];
+  auto entry = [&](int u, int tv, int& p, int& t) __attribute__((always_inline)) {
+    const int k = min(u * NT + tv, total - 1);
+    p = P > 1 ? (int)__umulhi((uint32_t)k, magic) : k;
+    t = k - p * P;
+  };
+  auto fetch_rows = [&](uint32_t c0) __attribute__((always_inline)) {
+    int tv = tid;
+    asm volatile("" : "+v"(tv));   // entry indices recomputed here, not kept in registers
 #pragma unroll
-    for (int t = 0; t < PMAX; ++t)
-      if (t < nmain) slotv[t * pairs + tid] = frow[t];
-  }
-  store();
-  __syncthreads();
-#pragma unroll 1
-  for (int ch = 0; ch < nchunks; ++ch) {
-    if (ch + 1 < nchunks) load(ch + 1);
-    const uint32_t r0 = (uint32_t)(ch * rows);
+    for (int u = 0; u < FB; ++u) {
+      int p, t;
+      entry(u, tv, p, t);
+      const uint32_t rp =
+          ((c0 + ((uint32_t)p & (SC - 1u))) * (uint32_t)atab[p >> log2sc]) & (B - 1u);
+      fv[u] = fTv[(size_t)rp * (size_t)P + (size_t)t];
+    }
+  };
+  // (a clamped entry is stored again with its own value: every load is known complete here)
+  auto put_rows = [&]() __attribute__((always_inline)) {
+    int tv = tid;
+    asm volatile("" : "+v"(tv));
+#pragma unroll
+    for (int u = 0; u < FB; ++u) {
+      int p, t;
+      entry(u, tv, p, t);
+      slotv[t * pairs + p] = fv[u];
+    }
+  };
+  uint32_t r = 0, ai = 0, base = 0, step = 0;
+  int nmain = 0;
+  auto pair_state = [&](uint32_t c0) __attribute__((always_inline)) {
     if (active) {
-      // the taps whose rows fall in this chunk (a bit per tap): slot t becomes x * f
-      uint32_t hits = 0;
+      ai = (uint32_t)atab[32 + l];
+      r = ((c0 + cc) * (uint32_t)atab[l]) & (B - 1u);
+      nmain = (!tiled && (int)r < residue) ? T + 1 : T;
+      base = (r * ai) & mask;
+      step = (B * ai) & mask;
+    }
+  };
+
+  // (a, ai) of every loop and the comb offsets into LDS: their loads go out before the first
+  // chunk's, so that waiting for them does not wait for the chunk
+  int av = 0, aiv = 0, ov = 0;
+  if (tid < loops) {
+    av = a_dev[tid];
+    aiv = ai_dev[tid];
+  }
+  if (tid < Rl) ov = offs[tid];
+  load(xa, 0);
+  if (tid < loops) {
+    atab[tid] = av;
+    atab[32 + tid] = aiv;
+  }
+  if (tid < Rl) atab[64 + tid] = ov;
+  __syncthreads();
+  fetch_rows(blockIdx.x << log2sc);
+  put_rows();
+  pair_state(blockIdx.x << log2sc);
+  store(xa);
+  __syncthreads();
+
+  // chunk q from LDS while chunk q + 1 is on its way.  Every load and store is issued on every
+  // call (the clamped chunk past the end again): no path on which a load could still be in
+  // flight, so the compiler adds no waits for one.
+  auto body = [&](int q) __attribute__((always_inline)) {
+    const int ch = q & (nchunks - 1);
+    const int sl = q >> log2nch;
+    const uint32_t c0 = (blockIdx.x + (uint32_t)sl * G) << log2sc;
+    const uint32_t cnext = (blockIdx.x + (uint32_t)min(sl + 1, nsl - 1) * G) << log2sc;
+    const uint32_t r0 = (uint32_t)(ch * rows);
+    // comb columns of the slab: this chunk's rows
+    for (int e = tid; e < ncopy; e += NT) {
+      const int row = e & (rows - 1);
+      const int ci = e >> log2rows;
+      const int rr = ci >> log2ncs;
+      const uint32_t col = c0 + (((uint32_t)atab[64 + rr] - c0) & smask) +
+                           ((uint32_t)(ci & ((1 << log2ncs) - 1)) << log2sigma);
+      if (col < c0 + SC) {
+        V2* __restrict__ dst = (V2*)yc + (size_t)rr * W +
+                               ((size_t)(col >> log2sigma) << log2m) + r0 + (uint32_t)row;
+        *dst = ldsv[row * (SL_PITCH * EPU) + (int)(col - c0)];
+      }
+    }
+    load(xa, q + 1);
+    if (active) {
+      // the taps whose rows fall in this chunk (a bit per tap): slot t becomes x * f (tap
+      // rows recomputed per chunk, not kept in registers)
+      uint32_t bs = base, sp = step;
+      int nm = nmain;
+      asm volatile("" : "+v"(bs), "+v"(sp), "+v"(nm));
+      uint32_t hits = 0;   // built from the top tap down (no per-tap bit constants)
 #pragma unroll
-      for (int t = 0; t < PMAX; ++t) {
-        const uint32_t row = ((base + (uint32_t)t * step) & mask) >> log2B;
-        hits |= (t < nmain && row - r0 < (uint32_t)rows) ? (1u << t) : 0u;
+      for (int t = PMAX - 1; t >= 0; --t) {
+        const uint32_t row = ((bs + (uint32_t)t * sp) & mask) >> log2B;
+        hits = (hits << 1) | (uint32_t)(t < nm && row - r0 < (uint32_t)rows);
       }
       while (hits) {
         const int t = __ffs(hits) - 1;
         hits &= hits - 1u;
-        const uint32_t row = ((base + (uint32_t)t * step) & mask) >> log2B;
+        const uint32_t row = ((bs + (uint32_t)t * sp) & mask) >> log2B;
         const V2 xv = ldsv[(row - r0) * (SL_PITCH * EPU) + cc];
-        const V2 fv = slotv[t * pairs + tid];
+        const V2 fw = slotv[t * pairs + tid];
         T2 xt, ft;
         xt.x = xv.x;
         xt.y = xv.y;
-        ft.x = fv.x;
-        ft.y = fv.y;
+        ft.x = fw.x;
+        ft.y = fw.y;
         const T2 term = fold_term(xt, ft);
         V2 tv;
         tv.x = term.x;
@@ -984,45 +1102,56 @@ __global__ __launch_bounds__(NT) void fold_slab_kernel(
         slotv[t * pairs + tid] = tv;
       }
     }
-    for (int rr = 0; rr < Rl; ++rr) {   // comb columns of the slab: this chunk's rows
-      const uint32_t o = (uint32_t)offs[rr];
-      for (uint32_t col = c0 + ((o - c0) & smask); col < c0 + SC; col += smask + 1u) {
-        V2* __restrict__ dst =
-            (V2*)yc + (size_t)rr * W + ((size_t)(col >> log2sigma) << log2m) + r0;
-        for (int row = tid; row < rows; row += NT)
-          dst[row] = ldsv[row * (SL_PITCH * EPU) + (col - c0)];
-      }
-    }
     __syncthreads();
-    if (ch + 1 < nchunks) {
-      store();
-      __syncthreads();
-    }
-  }
-  if (!active) return;
-  AC acc;
-  acc.init();
+    if (ch == nchunks - 1) {
+      // the slab is done: the next slab's filter rows go out (the current slab's again after
+      // the workgroup's last slab), this slab's buckets are summed and written while they
+      // arrive
+      fetch_rows(cnext);
+      if (active) {
+        int tv = tid;
+        asm volatile("" : "+v"(tv));   // slot addresses recomputed here
+        AC acc;
+        acc.init();
+        // the slots in batches of SB reads in flight together (clamped), added up in order
+        constexpr int SB = sizeof(Real) == 8 ? 4 : 8;
 #pragma unroll
-  for (int t = 0; t < PMAX; ++t) {
-    if (t < nmain) {
-      const V2 tv = slotv[t * pairs + tid];
-      T2 term;
-      term.x = tv.x;
-      term.y = tv.y;
-      acc.accumulate(term);
+        for (int t0 = 0; t0 < PMAX; t0 += SB) {
+          V2 sv[SB];
+#pragma unroll
+          for (int j = 0; j < SB; ++j) sv[j] = slotv[min(t0 + j, P - 1) * pairs + tv];
+#pragma unroll
+          for (int j = 0; j < SB; ++j) {
+            if (t0 + j < nmain) {
+              T2 term;
+              term.x = sv[j].x;
+              term.y = sv[j].y;
+              acc.accumulate(term);
+            }
+          }
+        }
+        if (tiled && (int)r < residue) {   // the reference's tail tap (Q2): another column
+          const uint32_t id = r * (uint32_t)T + B;
+          acc.add(x[(id * ai) & mask], fT[(size_t)r * (size_t)P + T]);
+        }
+        bins[(size_t)l * B + c0 + cc] = acc.get();   // column order: whole segments
+      }
+      __syncthreads();   // every slot read
+      put_rows();
+      pair_state(cnext);
     }
-  }
-  if (tiled && (int)r < residue) {   // the reference's tail tap (Q2): another column
-    const uint32_t id = r * (uint32_t)T + B;
-    acc.add(x[(id * ai) & mask], fT[(size_t)r * (size_t)P + T]);
-  }
-  bins[(size_t)l * B + r] = acc.get();
+    store(xa);
+    __syncthreads();
+  };
+#pragma unroll 1
+  for (int q = 0; q < Q; ++q) body(q);
 }
 
 int fold_rows_pitch(int B, int w) { return w / B + 1; }
 
 static size_t slab_lds_bytes(int is_fp32, int loops, int lsc, int P) {
-  return (size_t)SL_ROWS * SL_PITCH * 16 + (size_t)P * ((size_t)loops << lsc) * (is_fp32 ? 8 : 16);
+  return (size_t)SL_ROWS * SL_PITCH * 16 + (size_t)P * ((size_t)loops << lsc) * (is_fp32 ? 8 : 16) +
+         SL_TABLE * sizeof(int);   // chunk, tap slots, (a, ai) of up to 32 loops, comb offsets
 }
 
 // columns per slab (log2) for the LDS-staged fold, or -1 when it does not apply: 256-byte row
@@ -1033,6 +1162,9 @@ static int slab_log2sc(int is_fp32, int log2B, int log2m, int loops, int P) {
   (void)log2m;
   if (lsc > log2B) return -1;
   if ((loops << lsc) > 512 || P > 32) return -1;
+  // every filter slot of a slab fetched by one round of loads
+  const int fb = is_fp32 ? SlabFetch<float>::fb : SlabFetch<double>::fb;
+  if (P * (loops << lsc) > fb * 512) return -1;
   if (slab_lds_bytes(is_fp32, loops, lsc, P) > 96 * 1024) return -1;
   return lsc;
 }
@@ -1043,8 +1175,18 @@ static hipError_t slab_launch(const void* x, const void* fT, void* bins, const i
                               int loops, int tiled, int T, int P, int lsc, void* yc,
                               const int* offs, int Rl, int log2sigma, hipStream_t st) {
   using T2 = typename CT<Real>::T2;
-  const unsigned blocks = 1u << (log2B - lsc);
+  const unsigned nslabs = 1u << (log2B - lsc);
   const size_t lds = slab_lds_bytes(sizeof(Real) == 4, loops, lsc, P);
+  // one workgroup per resident slot (each streams nslabs / grid slabs); any grid is correct
+  int dev = 0, cus = 0, per_cu = 0;
+  unsigned blocks = nslabs;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void*>(&fold_slab_kernel<Real, PMAX, NT>), NT, lds) ==
+          hipSuccess &&
+      cus > 0 && per_cu > 0)
+    blocks = std::min(nslabs, (unsigned)(cus * per_cu));
   hipLaunchKernelGGL((fold_slab_kernel<Real, PMAX, NT>), dim3(blocks), dim3(NT), lds, st,
                      (const T2*)x, (const T2*)fT, (T2*)bins, a_dev, ai_dev, mask, log2B, log2m,
                      w, loops, tiled, T, P, lsc, (T2*)yc, offs, Rl, log2sigma);
@@ -1100,8 +1242,8 @@ hipError_t filter_rows_launch(int is_fp32, const void* f, void* fT, int B, int w
 
 hipError_t fold_columns_launch(int is_fp32, const void* x, const void* fT, void* bins,
                                const int* a_dev, const int* ai_dev, int n, int B, int w,
-                               int loops, int tiled, int remap, void* yc, const int* offs,
-                               int Rl, int W, hipStream_t st) {
+                               int loops, int tiled, int by_column, int remap, void* yc,
+                               const int* offs, int Rl, int W, hipStream_t st) {
   int log2B = 0, log2n = 0, log2W = 0;
   while ((1 << log2B) < B) ++log2B;
   while ((1LL << log2n) < n) ++log2n;
@@ -1110,12 +1252,14 @@ hipError_t fold_columns_launch(int is_fp32, const void* x, const void* fT, void*
   if (!yc || log2sigma > log2B) Rl = 0;   // no comb columns to copy (sigma must divide B)
   const int T = w / B, P = fold_rows_pitch(B, w);
   const uint32_t mask = (uint32_t)n - 1u;
-  const int lsc = remap >= 0 ? slab_log2sc(is_fp32, log2B, log2m, loops, P) : -1;
-  if (lsc >= 0)
+  if (by_column) {
+    const int lsc = Rl <= 64 ? slab_log2sc(is_fp32, log2B, log2m, loops, P) : -1;
+    if (lsc < 0) return hipErrorInvalidValue;
     return is_fp32 ? slab_dispatch<float>(P, x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w,
                                           loops, tiled, T, lsc, yc, offs, Rl, log2sigma, st)
                    : slab_dispatch<double>(P, x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w,
                                            loops, tiled, T, lsc, yc, offs, Rl, log2sigma, st);
+  }
   // columns per block: one 128-byte line of x (8 complex128 / 16 complex64), fewer if B is
   // smaller
   int log2lw = is_fp32 ? 4 : 3;

@@ -39,7 +39,7 @@ struct FftTables {
 hipError_t fft_tables_create(int log2n, int inverse, int is_fp32, FftTables* t);
 void fft_tables_destroy(FftTables* t);
 
-enum FftLoad { kLoadPlain = 0, kLoadComb = 1, kLoadCombRows = 2 };
+enum FftLoad { kLoadPlain = 0, kLoadComb = 1, kLoadCombRows = 2, kLoadPerm = 3 };
 enum FftStore { kStoreCplx = 0, kStoreKeys = 1, kStoreBoth = 2 };
 
 // One canonical FFT of `batch` transforms.  Passes ping-pong between out and scratch so that
@@ -64,6 +64,8 @@ struct FftLaunch {
   int comb_sigma = 1;
   int comb_log2nlo = 0;          // kLoadCombRows: element g of batch b at
   int comb_log2m = 0;            // in[b*in_bstride + (g mod 2^log2nlo) << log2m | g >> log2nlo]
+  const int* perm = nullptr;     // kLoadPerm: element g of batch b at
+                                 // in[b*in_bstride + (g * perm[b] mod N)] (perm[b] odd)
   int qmax = 0;                  // most columns per tile (power of two; 0 = the default)
   int keys_tiled = 0;            // keys in the last pass's tile order (see KeyMap)
   uint16_t* digits = nullptr;    // optional: each key's top 12 bits below the sign bit (the
@@ -93,21 +95,23 @@ hipError_t perm_filter_launch(int is_fp32, const void* x, const void* f, void* b
                               const int* ai_dev, int n, int B, int w, int loops, int tiled,
                               hipStream_t st);
 // column-ordered fold (same sums as perm_filter_launch): fT is the bucket-major filter copy
-// filter_rows_launch writes (B rows of fold_rows_pitch(B, w) taps); a_dev and ai_dev per loop;
-// The slab fold (LDS-staged, the whole signal streamed once) runs where a 64-byte-wide slab of
-// all n/B rows fits 64 KB of LDS; elsewhere, or with remap < 0, each thread gathers its taps
-// from global memory (remap: XCD-aware workgroup order).  With yc, the comb
-// samples of the Rl offsets offs[] are copied too, column-major: yc[r*W + lo*m + row] =
+// filter_rows_launch writes (B rows of fold_rows_pitch(B, w) taps); a_dev and ai_dev per loop.
+// by_column = 1: the slab fold (LDS-staged, the whole signal streamed once; only where
+// fold_slab_applies, else hipErrorInvalidValue) writing the bucket sums in column order,
+// bins[l*B + c] = sum of bucket (c * a_l mod B), for a bucket FFT that loads with kLoadPerm
+// (perm = ai).  by_column = 0: each thread gathers its taps from global memory (remap:
+// XCD-aware workgroup order) and writes bucket order.  With yc, the comb samples of the Rl
+// offsets offs[] are copied too, column-major: yc[r*W + lo*m + row] =
 // x[off_r + (row*(B/sigma) + lo)*sigma] (m = n/B, sigma = n/W; needs sigma <= B)
 int fold_rows_pitch(int B, int w);
-// whether the LDS-staged slab fold takes this shape (else the per-thread column gathers)
+// whether the LDS-staged slab fold takes this shape (at most 64 comb loops)
 int fold_slab_applies(int is_fp32, int n, int B, int w, int loops);
 hipError_t filter_rows_launch(int is_fp32, const void* f, void* fT, int B, int w, int tiled,
                               hipStream_t st);
 hipError_t fold_columns_launch(int is_fp32, const void* x, const void* fT, void* bins,
                                const int* a_dev, const int* ai_dev, int n, int B, int w,
-                               int loops, int tiled, int remap, void* yc, const int* offs,
-                               int Rl, int W, hipStream_t st);
+                               int loops, int tiled, int by_column, int remap, void* yc,
+                               const int* offs, int Rl, int W, hipStream_t st);
 // same fold, reading the tap values from the staging array T[l*w + i] of the streaming pass
 // batch: one persistent launch over the permute+filter of several signals
 struct PfJob {

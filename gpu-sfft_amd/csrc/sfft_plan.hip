@@ -258,6 +258,7 @@ extern "C" sfft_status sfft_plan_create(const sfft_params* prm, sfft_precision p
   P->columns = prm->front_end == SFFT_FRONT_COLUMNS ||
                (prm->front_end == SFFT_FRONT_AUTO && P->fp32 &&
                 fold_slab_applies(1, prm->n, prm->B, prm->w, L));
+  P->bins_perm = P->columns && fold_slab_applies(P->fp32, prm->n, prm->B, prm->w, L);
   if (P->stream) {
     items.push_back({&P->taps, (size_t)L * prm->w * e});
     items.push_back({&P->comb_y, (size_t)Rl * W * e});
@@ -513,8 +514,8 @@ sfft_status run_fold(sfft_plan* P, const void* d_x, hipStream_t st) {
                            tiled, st));
   else if (P->columns)
     HIPCK(fold_columns_launch(P->fp32, d_x, P->frows, P->bins_t, P->d_a, P->d_ai, p.n, p.B, p.w,
-                              p.loops, tiled, 1, P->comb_y, P->d_offs, p.RLFC_loops, p.W_RLFC,
-                              st));
+                              p.loops, tiled, P->bins_perm, 1, P->comb_y, P->d_offs,
+                              p.RLFC_loops, p.W_RLFC, st));
   else
     HIPCK(perm_filter_launch(P->fp32, d_x, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, p.loops,
                              tiled, st));
@@ -540,6 +541,10 @@ sfft_status run_bucket(sfft_plan* P, hipStream_t st, bool marks) {
   f.batch = L;
   f.store = kStoreBoth;
   f.qmax = P->qmax_B;
+  if (P->bins_perm) {   // bucket r of loop l at column r * ai_l mod B
+    f.load = kLoadPerm;
+    f.perm = P->d_ai;
+  }
   HIPCK(fft_run(f, st));
   // ascending bucket lists: the hit list is ordered by (loop, bucket, walk)
   HIPCK(select_topk(P->fp32, P->bkeys, B, (int)B, K, Ll, P->J, 1, P->selB, st, P->jsel));

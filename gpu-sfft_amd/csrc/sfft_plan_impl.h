@@ -28,6 +28,7 @@ struct sfft_plan {
   void* taps = nullptr;         // loops x w tap values (stream)
   void* comb_y = nullptr;       // RLFC_loops x W comb samples (stream)
   int columns = 0;              // column-ordered fold (SFFT_FRONT_COLUMNS)
+  int bins_perm = 0;            // the slab fold: bins_t in column order, read via kLoadPerm
   void* frows = nullptr;        // bucket-major filter taps of the column fold
   void* bins_t = nullptr;
   void* bins_scr = nullptr;

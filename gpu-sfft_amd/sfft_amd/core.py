@@ -281,7 +281,9 @@ class Plan:
 
     def debug_array(self, which, count, dtype):
         """Copies `count` items of a plan-owned device buffer to a numpy array
-        (0 bins_f, 1 J, 2 J2 lists, 3 hits, 4 bins_t, 5 nodes, 6 count, 7 sel, 8 comb keys)."""
+        (0 bins_f, 1 J, 2 J2 lists, 3 hits, 4 bins_t, 5 nodes, 6 count, 7 sel, 8 comb keys).
+        bins_t is in bucket order, or in column order (bucket c * a_l mod B at c) after the
+        column front end's slab fold."""
         torch = _torch()
         ptr = lib.sfft_plan_debug_buffer(self._h, which)
         out = np.empty(count, dtype=dtype)
