@@ -124,6 +124,7 @@ struct PassArgs {
   int log2N, r, log2Lprev, log2Q;
   int inverse;
   const void* twR;
+  const void* tw4;      // radix-4 stage twiddle quads (see fft_tables_create)
   const void* tdir;
   const void* thi;
   const void* tlo;
@@ -152,28 +153,45 @@ __device__ __forceinline__ T2 outer_tw(const PassArgs& A, uint32_t e, int log2Lc
 
 template <typename T2, int EPT, int NT>
 __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol, int log2ns,
-                                           const T2* __restrict__ twR, int inverse) {
+                                           const T2* __restrict__ tw4, int inverse) {
+  // Butterfly indices past the tile (partial tiles) are clamped to the last butterfly, whose
+  // thread stores the same values: no per-butterfly branches, so every LDS read and twiddle
+  // load of the thread is in flight together.  Twiddled (ns > 1) and plain stages are separate
+  // straight-line copies.
   const int tid = threadIdx.x;
   const int Rq = 1 << (r - 2);
   const int nbf = (1 << log2ncol) * Rq;
   const int ns = 1 << log2ns;
-  const int tstep = 1 << (r - log2ns - 2);
   T2 v[EPT];
+  if (ns > 1) {
+    const T2* __restrict__ q4 = tw4 + 4 * ((ns - 4) / 3);   // this stage's quads
 #pragma unroll
-  for (int kk = 0; kk < EPT / 4; ++kk) {
-    const int bi = tid + kk * NT;
-    if (bi < nbf) {
+    for (int kk = 0; kk < EPT / 4; ++kk) {
+      const int bi = min(tid + kk * NT, nbf - 1);
       const int c = bi >> (r - 2);
       const int j = bi & (Rq - 1);
       const T2* col = lds + c * RP;
       T2 a0 = col[LPAD(j)], a1 = col[LPAD(j + Rq)], a2 = col[LPAD(j + 2 * Rq)],
          a3 = col[LPAD(j + 3 * Rq)];
-      if (ns > 1) {
-        const int jm = j & (ns - 1);
-        a1 = cmul(a1, twR[jm * tstep]);
-        a2 = cmul(a2, twR[2 * jm * tstep]);
-        a3 = cmul(a3, twR[3 * jm * tstep]);
-      }
+      const T2* w = q4 + 4 * (j & (ns - 1));   // {W^jm, W^2jm, W^3jm, 0}
+      a1 = cmul(a1, w[0]);
+      a2 = cmul(a2, w[1]);
+      a3 = cmul(a3, w[2]);
+      bfly4(a0, a1, a2, a3, inverse);
+      v[4 * kk + 0] = a0;
+      v[4 * kk + 1] = a1;
+      v[4 * kk + 2] = a2;
+      v[4 * kk + 3] = a3;
+    }
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < EPT / 4; ++kk) {
+      const int bi = min(tid + kk * NT, nbf - 1);
+      const int c = bi >> (r - 2);
+      const int j = bi & (Rq - 1);
+      const T2* col = lds + c * RP;
+      T2 a0 = col[LPAD(j)], a1 = col[LPAD(j + Rq)], a2 = col[LPAD(j + 2 * Rq)],
+         a3 = col[LPAD(j + 3 * Rq)];
       bfly4(a0, a1, a2, a3, inverse);
       v[4 * kk + 0] = a0;
       v[4 * kk + 1] = a1;
@@ -184,17 +202,15 @@ __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol,
   __syncthreads();
 #pragma unroll
   for (int kk = 0; kk < EPT / 4; ++kk) {
-    const int bi = tid + kk * NT;
-    if (bi < nbf) {
-      const int c = bi >> (r - 2);
-      const int j = bi & (Rq - 1);
-      T2* col = lds + c * RP;
-      const int base = ((j >> log2ns) << (log2ns + 2)) + (j & (ns - 1));
-      col[LPAD(base)] = v[4 * kk + 0];
-      col[LPAD(base + ns)] = v[4 * kk + 1];
-      col[LPAD(base + 2 * ns)] = v[4 * kk + 2];
-      col[LPAD(base + 3 * ns)] = v[4 * kk + 3];
-    }
+    const int bi = min(tid + kk * NT, nbf - 1);
+    const int c = bi >> (r - 2);
+    const int j = bi & (Rq - 1);
+    T2* col = lds + c * RP;
+    const int base = ((j >> log2ns) << (log2ns + 2)) + (j & (ns - 1));
+    col[LPAD(base)] = v[4 * kk + 0];
+    col[LPAD(base + ns)] = v[4 * kk + 1];
+    col[LPAD(base + 2 * ns)] = v[4 * kk + 2];
+    col[LPAD(base + 3 * ns)] = v[4 * kk + 3];
   }
   __syncthreads();
 }
@@ -202,21 +218,33 @@ __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol,
 template <typename T2, int EPT, int NT>
 __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol, int log2ns,
                                            const T2* __restrict__ twR) {
+  // (clamped butterfly indices as in fft_stage4)
   const int tid = threadIdx.x;
   const int Rh = 1 << (r - 1);
   const int nbf = (1 << log2ncol) * Rh;
   const int ns = 1 << log2ns;
   const int tstep = 1 << (r - log2ns - 1);
   T2 v[EPT];
+  if (ns > 1) {
 #pragma unroll
-  for (int kk = 0; kk < EPT / 2; ++kk) {
-    const int bi = tid + kk * NT;
-    if (bi < nbf) {
+    for (int kk = 0; kk < EPT / 2; ++kk) {
+      const int bi = min(tid + kk * NT, nbf - 1);
       const int c = bi >> (r - 1);
       const int j = bi & (Rh - 1);
       const T2* col = lds + c * RP;
       T2 a0 = col[LPAD(j)], a1 = col[LPAD(j + Rh)];
-      if (ns > 1) a1 = cmul(a1, twR[(j & (ns - 1)) * tstep]);
+      a1 = cmul(a1, twR[(j & (ns - 1)) * tstep]);
+      v[2 * kk + 0] = cadd(a0, a1);
+      v[2 * kk + 1] = csub(a0, a1);
+    }
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < EPT / 2; ++kk) {
+      const int bi = min(tid + kk * NT, nbf - 1);
+      const int c = bi >> (r - 1);
+      const int j = bi & (Rh - 1);
+      const T2* col = lds + c * RP;
+      const T2 a0 = col[LPAD(j)], a1 = col[LPAD(j + Rh)];
       v[2 * kk + 0] = cadd(a0, a1);
       v[2 * kk + 1] = csub(a0, a1);
     }
@@ -224,20 +252,22 @@ __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol,
   __syncthreads();
 #pragma unroll
   for (int kk = 0; kk < EPT / 2; ++kk) {
-    const int bi = tid + kk * NT;
-    if (bi < nbf) {
-      const int c = bi >> (r - 1);
-      const int j = bi & (Rh - 1);
-      T2* col = lds + c * RP;
-      const int base = ((j >> log2ns) << (log2ns + 1)) + (j & (ns - 1));
-      col[LPAD(base)] = v[2 * kk + 0];
-      col[LPAD(base + ns)] = v[2 * kk + 1];
-    }
+    const int bi = min(tid + kk * NT, nbf - 1);
+    const int c = bi >> (r - 1);
+    const int j = bi & (Rh - 1);
+    T2* col = lds + c * RP;
+    const int base = ((j >> log2ns) << (log2ns + 1)) + (j & (ns - 1));
+    col[LPAD(base)] = v[2 * kk + 0];
+    col[LPAD(base + ns)] = v[2 * kk + 1];
   }
   __syncthreads();
 }
 
-template <typename Real, int LOAD, int STORE>
+// TW: the pass's outer twiddles, known at compile time so that the load phase is straight-line
+// code: 0 none (the first pass), 1 direct table entries, 2 products of a high and a low entry
+enum PassTw { kTwNone = 0, kTwDirect = 1, kTwSplit = 2 };
+
+template <typename Real, int LOAD, int STORE, int TW>
 // 8 waves per SIMD for the 1024-thread tiles: two workgroups per CU
 __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void fft_pass_kernel(
     PassArgs A) {
@@ -267,7 +297,7 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
   const int log2Lcur = log2Lprev + r;
   const long long Lprev = 1LL << log2Lprev;
   const long long nq = 1LL << (A.log2N - log2Lcur);
-  const bool first = (log2Lprev == 0);
+  constexpr bool first = (TW == kTwNone);   // log2Lprev == 0
   int log2ncol;
   long long q0 = 0, q = 0, t0 = 0;
   if (first) {
@@ -281,6 +311,10 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
   }
   const int ncol = 1 << log2ncol;
   const int TE = ncol << r;
+  // element (c, v) of the tile: first pass g = q0 + c + nq*v, later passes
+  // g = (q + nq*v)*Lprev + t0 + c -- both gbase + c + vstride*v (no per-element branches)
+  const long long gbase = first ? q0 : q * Lprev + t0;
+  const long long vstride = first ? nq : nq * Lprev;
 
   // ---- load (+ outer twiddle) into LDS, column c at lds[c*RP + v].  Every element load of
   // the thread (and every twiddle load) is issued before any is used: in one basic block,
@@ -295,7 +329,7 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
   // 16 bytes x 2 (FP64), which keeps them within 64 registers (two 1024-thread workgroups
   // per CU); the comb-gathering pass issues all of its loads together
   constexpr int LB = (LOAD == kLoadComb) ? EPT : (sizeof(Real) == 8 ? 2 : 4);
-  const bool split = log2Lcur > kMaxPassLog;
+  constexpr bool split = (TW == kTwSplit);   // log2Lcur > kMaxPassLog
 #pragma unroll
   for (int kb = 0; kb < EPT; kb += LB) {
     V2 vv[LB];
@@ -304,11 +338,7 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
       const int e = min(tid + (kb + k) * NT, TE - 1);
       const int c = e & (ncol - 1);
       const int v = e >> log2ncol;
-      long long g;
-      if (first)
-        g = q0 + c + nq * v;
-      else
-        g = (q + nq * v) * Lprev + t0 + c;
+      const long long g = gbase + c + vstride * v;
       const T2* src;
       if (LOAD == kLoadComb)
         src = cx + g * (long long)A.comb_sigma;
@@ -320,11 +350,11 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
         src = in + g;
       vv[k] = *(const V2*)src;
     }
-    if (!first) {
+    if constexpr (!first) {
       // outer twiddles W_Lcur^((t0+c)*v): a direct table entry, or the product of a high and
       // a low table entry (every table load of the batch issued first, as for the data)
       V2 tw[LB], tl[LB];
-      if (split) {
+      if constexpr (split) {
         const V2* __restrict__ thi = (const V2*)A.thi;
         const V2* __restrict__ tlo = (const V2*)A.tlo;
         const uint32_t lmask = (1u << A.s) - 1u;
@@ -357,31 +387,29 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
       }
 #pragma unroll
       for (int k = 0; k < LB; ++k) {
-        const int e = tid + (kb + k) * NT;
-        if (e < TE) {
-          const int c = e & (ncol - 1);
-          const int v = e >> log2ncol;
-          T2 val, w;
-          val.x = vv[k].x;
-          val.y = vv[k].y;
-          w.x = tw[k].x;
-          w.y = tw[k].y;
-          if (v > 0) val = cmul(val, w);
-          lds[c * RP + LPAD(v)] = val;
-        }
+        // (a clamped element is stored again with its own value: no branch)
+        const int e = min(tid + (kb + k) * NT, TE - 1);
+        const int c = e & (ncol - 1);
+        const int v = e >> log2ncol;
+        T2 val, w;
+        val.x = vv[k].x;
+        val.y = vv[k].y;
+        w.x = tw[k].x;
+        w.y = tw[k].y;
+        const T2 tv = cmul(val, w);
+        if (v > 0) val = tv;
+        lds[c * RP + LPAD(v)] = val;
       }
     } else {
 #pragma unroll
       for (int k = 0; k < LB; ++k) {
-        const int e = tid + (kb + k) * NT;
-        if (e < TE) {
-          const int c = e & (ncol - 1);
-          const int v = e >> log2ncol;
-          T2 val;
-          val.x = vv[k].x;
-          val.y = vv[k].y;
-          lds[c * RP + LPAD(v)] = val;
-        }
+        const int e = min(tid + (kb + k) * NT, TE - 1);
+        const int c = e & (ncol - 1);
+        const int v = e >> log2ncol;
+        T2 val;
+        val.x = vv[k].x;
+        val.y = vv[k].y;
+        lds[c * RP + LPAD(v)] = val;
       }
     }
   }
@@ -389,10 +417,11 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
 
   // ---- sub-FFT of every column: radix-4 stages, then one radix-2 stage if r is odd
   const T2* __restrict__ twR = (const T2*)A.twR;
+  const T2* __restrict__ tw4 = (const T2*)A.tw4;
   int log2ns = 0;
   int rem = r;
   while (rem >= 2) {
-    fft_stage4<T2, EPT, NT>(lds, RP, r, log2ncol, log2ns, twR, A.inverse);
+    fft_stage4<T2, EPT, NT>(lds, RP, r, log2ncol, log2ns, tw4, A.inverse);
     log2ns += 2;
     rem -= 2;
   }
@@ -431,41 +460,39 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
   }
 }
 
-template <typename Real, int LOAD, int STORE>
+template <typename Real, int LOAD, int STORE, int TW>
 static hipError_t launch_pass(const PassArgs& A, int tiles, int batch, hipStream_t st) {
   const dim3 grid = (LOAD == kLoadComb) ? dim3(tiles * batch, 1) : dim3(tiles, batch);
-  hipLaunchKernelGGL((fft_pass_kernel<Real, LOAD, STORE>), grid, dim3(Tile<Real, LOAD>::nt), 0,
-                     st, A);
+  hipLaunchKernelGGL((fft_pass_kernel<Real, LOAD, STORE, TW>), grid, dim3(Tile<Real, LOAD>::nt),
+                     0, st, A);
   return hipGetLastError();
 }
 
-template <typename Real>
-static hipError_t dispatch_pass(const PassArgs& A, int load, int store, int tiles, int batch,
-                                hipStream_t st) {
-  if (load == kLoadCombRows) {
-    if (store == kStoreCplx)
-      return launch_pass<Real, kLoadCombRows, kStoreCplx>(A, tiles, batch, st);
-    if (store == kStoreKeys)
-      return launch_pass<Real, kLoadCombRows, kStoreKeys>(A, tiles, batch, st);
-    return launch_pass<Real, kLoadCombRows, kStoreBoth>(A, tiles, batch, st);
-  }
-  if (load == kLoadPerm) {
-    if (store == kStoreCplx) return launch_pass<Real, kLoadPerm, kStoreCplx>(A, tiles, batch, st);
-    if (store == kStoreKeys) return launch_pass<Real, kLoadPerm, kStoreKeys>(A, tiles, batch, st);
-    return launch_pass<Real, kLoadPerm, kStoreBoth>(A, tiles, batch, st);
-  }
-  if (load == kLoadComb) {
-    if (store == kStoreCplx) return launch_pass<Real, kLoadComb, kStoreCplx>(A, tiles, batch, st);
-    if (store == kStoreKeys) return launch_pass<Real, kLoadComb, kStoreKeys>(A, tiles, batch, st);
-    return launch_pass<Real, kLoadComb, kStoreBoth>(A, tiles, batch, st);
-  }
-  if (store == kStoreCplx) return launch_pass<Real, kLoadPlain, kStoreCplx>(A, tiles, batch, st);
-  if (store == kStoreKeys) return launch_pass<Real, kLoadPlain, kStoreKeys>(A, tiles, batch, st);
-  return launch_pass<Real, kLoadPlain, kStoreBoth>(A, tiles, batch, st);
+template <typename Real, int LOAD, int TW>
+static hipError_t dispatch_store(const PassArgs& A, int store, int tiles, int batch,
+                                 hipStream_t st) {
+  if (store == kStoreCplx) return launch_pass<Real, LOAD, kStoreCplx, TW>(A, tiles, batch, st);
+  if (store == kStoreKeys) return launch_pass<Real, LOAD, kStoreKeys, TW>(A, tiles, batch, st);
+  return launch_pass<Real, LOAD, kStoreBoth, TW>(A, tiles, batch, st);
 }
 
-// log2 of the columns per tile of pass p (r bits, after log2Lprev bits): as many as fit the
-// tile, up to qmax; it does not change the arithmetic
+// the special loads only occur in first passes (no outer twiddles)
+template <typename Real>
+static hipError_t dispatch_pass(const PassArgs& A, int load, int store, int tw, int tiles,
+                                int batch, hipStream_t st) {
+  if (load == kLoadCombRows)
+    return dispatch_store<Real, kLoadCombRows, kTwNone>(A, store, tiles, batch, st);
+  if (load == kLoadPerm)
+    return dispatch_store<Real, kLoadPerm, kTwNone>(A, store, tiles, batch, st);
+  if (load == kLoadComb)
+    return dispatch_store<Real, kLoadComb, kTwNone>(A, store, tiles, batch, st);
+  if (tw == kTwSplit)
+    return dispatch_store<Real, kLoadPlain, kTwSplit>(A, store, tiles, batch, st);
+  if (tw == kTwDirect)
+    return dispatch_store<Real, kLoadPlain, kTwDirect>(A, store, tiles, batch, st);
+  return dispatch_store<Real, kLoadPlain, kTwNone>(A, store, tiles, batch, st);
+}
+
 static int pass_log2q(const FftTables& T, const FftLaunch& L, int p, int r, int log2Lprev) {
   const int qmax = (L.qmax > 0 && L.qmax <= FFT_QMAX) ? L.qmax : FFT_QMAX;
   const int load = (p == 0) ? L.load : kLoadPlain;
@@ -531,6 +558,7 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.log2Q = log2Q;
     A.inverse = T.inverse;
     A.twR = T.twR[p];
+    A.tw4 = T.tw4[p];
     A.tdir = T.tdir[p];
     A.thi = T.thi[p];
     A.tlo = T.tlo[p];
@@ -545,8 +573,10 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.keys_tiled = (last && p > 0 && L.keys_tiled) ? 1 : 0;
     A.digits = last ? L.digits : nullptr;
     const int store = last ? L.store : kStoreCplx;
-    hipError_t e = T.is_fp32 ? dispatch_pass<float>(A, load, store, tiles, L.batch, st)
-                             : dispatch_pass<double>(A, load, store, tiles, L.batch, st);
+    const int tw = (p == 0) ? kTwNone : (log2Lcur > kMaxPassLog ? kTwSplit : kTwDirect);
+    if (load != kLoadPlain && tw != kTwNone) return hipErrorInvalidValue;
+    hipError_t e = T.is_fp32 ? dispatch_pass<float>(A, load, store, tw, tiles, L.batch, st)
+                             : dispatch_pass<double>(A, load, store, tw, tiles, L.batch, st);
     if (e != hipSuccess) return e;
     cur_in = dst;
     cur_in_stride = dst_stride;

@@ -31,6 +31,7 @@ struct FftTables {
   const void* tdir[kMaxPasses] = {nullptr, nullptr, nullptr, nullptr};  // Lcur entries
   const void* thi[kMaxPasses] = {nullptr, nullptr, nullptr, nullptr};   // Lcur >> s
   const void* tlo[kMaxPasses] = {nullptr, nullptr, nullptr, nullptr};   // 1 << s
+  const void* tw4[kMaxPasses] = {nullptr, nullptr, nullptr, nullptr};   // radix-4 quads
   int s[kMaxPasses] = {0, 0, 0, 0};
   void* storage = nullptr;  // one device allocation holding every table
   size_t bytes = 0;

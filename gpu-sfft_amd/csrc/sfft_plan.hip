@@ -68,8 +68,11 @@ hipError_t fft_tables_create(int log2n, int inverse, int is_fp32, FftTables* t) 
   t->is_fp32 = is_fp32;
   t->npass = fft_plan_passes(log2n, t->logs);
   const size_t esz = is_fp32 ? 8 : 16;
-  // layout: per pass [twR (R)] [tdir (Lcur) | thi (Lcur>>s) tlo (1<<s)]
-  size_t cnt[kMaxPasses][4] = {};
+  // layout: per pass [twR (R)] [tdir (Lcur) | thi (Lcur>>s) tlo (1<<s)] [tw4]; tw4: for
+  // each radix-4 stage with ns = 4^k > 1 (k = 1, 2, ... while 2k <= r - 2), ns quads
+  // {twR[jm*t], twR[2*jm*t], twR[3*jm*t], 0} (t = R / (4*ns), jm < ns), stage k at quad
+  // (4^k - 4) / 3: a stage's three twiddles of neighbouring butterflies are neighbours
+  size_t cnt[kMaxPasses][5] = {};
   size_t total = 0;
   int log2Lprev = 0;
   for (int p = 0; p < t->npass; ++p) {
@@ -85,7 +88,8 @@ hipError_t fft_tables_create(int log2n, int inverse, int is_fp32, FftTables* t) 
         cnt[p][3] = (size_t)1 << t->s[p];
       }
     }
-    for (int j = 0; j < 4; ++j) total += cnt[p][j];
+    for (int k = 1; 2 * k <= r - 2; ++k) cnt[p][4] += (size_t)4 << (2 * k);
+    for (int j = 0; j < 5; ++j) total += cnt[p][j];
     log2Lprev = log2Lcur;
   }
   t->bytes = total * esz;
@@ -104,8 +108,12 @@ hipError_t fft_tables_create(int log2n, int inverse, int is_fp32, FftTables* t) 
     }
     off += esz;
   };
+  auto put_zero = [&]() {
+    memset(&host[off], 0, esz);
+    off += esz;
+  };
   log2Lprev = 0;
-  size_t start[kMaxPasses][4] = {};
+  size_t start[kMaxPasses][5] = {};
   for (int p = 0; p < t->npass; ++p) {
     const int r = t->logs[p];
     const int log2Lcur = log2Lprev + r;
@@ -118,6 +126,16 @@ hipError_t fft_tables_create(int log2n, int inverse, int is_fp32, FftTables* t) 
     for (size_t e = 0; e < cnt[p][2]; ++e) put((uint64_t)e << t->s[p], Lcur);
     start[p][3] = off;
     for (size_t e = 0; e < cnt[p][3]; ++e) put(e, Lcur);
+    start[p][4] = off;
+    for (int k = 1; 2 * k <= r - 2; ++k) {
+      const uint64_t ns = 1ull << (2 * k), ts = R / (4 * ns);
+      for (uint64_t jm = 0; jm < ns; ++jm) {
+        put(jm * ts, R);
+        put(2 * jm * ts, R);
+        put(3 * jm * ts, R);
+        put_zero();
+      }
+    }
     log2Lprev = log2Lcur;
   }
   void* d = nullptr;
@@ -137,6 +155,7 @@ hipError_t fft_tables_create(int log2n, int inverse, int is_fp32, FftTables* t) 
     t->tdir[p] = cnt[p][1] ? base + start[p][1] : nullptr;
     t->thi[p] = cnt[p][2] ? base + start[p][2] : nullptr;
     t->tlo[p] = cnt[p][3] ? base + start[p][3] : nullptr;
+    t->tw4[p] = cnt[p][4] ? base + start[p][4] : nullptr;
   }
   return hipSuccess;
 }
