@@ -150,9 +150,8 @@ def run_precision(args, S, prm, prec, seeds_draws, snr, log):
     if fp32:
         ft = S.to_complex64(ft)
     log(f"[bench] {prec}: filter design (FP64, {args.design}, w={w}) {time.time() - t:.2f}s")
-    # the plan's AUTO choice: column front end for FP32, gathers for FP64
-    front = args.front_end if args.front_end != "auto" else ("columns" if fp32 else "gather")
-    plan = S.Plan(prm, prec, front_end=front)
+    plan = S.Plan(prm, prec, front_end=args.front_end)
+    front = plan.front_end   # AUTO resolved by the plan
     plan.set_filter(ft, win)
     if args.launch == "graph":
         plan.set_graph(True)
@@ -194,20 +193,28 @@ def run_precision(args, S, prm, prec, seeds_draws, snr, log):
 
 
 def kernel_roofline(prm, prec, front, iv_ms, config):
-    """The dominant kernel's rate: algorithmic bytes per launch (inputs read once, outputs
-    written once; a gathered tap charged one 64-byte granule, at most the whole signal) over
-    its event-timed duration."""
+    """The dominant kernel's rate: algorithmic bytes per launch over its event-timed duration.
+    Column slab fold: the signal read once, the filter once, the bins and the comb samples
+    written once.  Gather fold: a gathered tap charged one 64-byte granule (at most the whole
+    signal), the filter once, the bins once.  Stream scatter: the signal once, the tap and comb
+    staging written once."""
     n, B, w, L = prm["n"], prm["B"], prm["w"], prm["loops"]
     e = 8 if prec == "fp32" else 16
     Wc, Rl = prm["W_RLFC"], prm["RLFC_loops"]
     kname, iv = FRONT_KERNEL[front]
     k_ms = iv_ms[iv]
+    extra = {}
     if front == "stream":
-        b64 = b128 = n * e + L * w * e + Rl * Wc * e
+        alg = n * e + L * w * e + Rl * Wc * e
+    elif front == "columns":
+        alg = n * e + w * e + L * B * e + Rl * Wc * e
     else:
-        b64 = min(L * w * GATHER_GRANULE, n * e) + w * e + L * B * e
+        alg = min(L * w * GATHER_GRANULE, n * e) + w * e + L * B * e
         b128 = min(L * w * LINE, n * e) + w * e + L * B * e
-    achieved = b64 / (k_ms * 1e-3) / 1e9
+        extra = {"granule_bytes": GATHER_GRANULE,
+                 "achieved_granule128": round(b128 / (k_ms * 1e-3) / 1e9, 1),
+                 "frac_granule128": round(b128 / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    achieved = alg / (k_ms * 1e-3) / 1e9
     traffic, tsrc = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
@@ -215,12 +222,12 @@ def kernel_roofline(prm, prec, front, iv_ms, config):
             entry = json.load(fh).get(f"{kname}_{config}_{prec}")
         if isinstance(entry, dict):
             traffic, tsrc = entry.get("bytes"), entry.get("source")
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "traffic_source": tsrc, "kernel": kname, "bytes_per_launch": b64,
-            "launch_ms": round(k_ms, 4), "granule_bytes": GATHER_GRANULE,
-            "achieved_granule128": round(b128 / (k_ms * 1e-3) / 1e9, 1),
-            "frac_granule128": round(b128 / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "traffic_source": tsrc, "kernel": kname, "bytes_per_launch": alg,
+           "launch_ms": round(k_ms, 4)}
+    out.update(extra)
+    return out
 
 
 def transform_roofline(prm, prec, ms, cnt):

@@ -151,53 +151,55 @@ __device__ __forceinline__ T2 outer_tw(const PassArgs& A, uint32_t e, int log2Lc
 // stores and the column-interleaved loads spread over the LDS banks.
 #define LPAD(x) ((x) + ((x) >> 4))
 
+// The twiddles of a stage's butterflies for this thread: radix-4 stage with ns = 4^k > 1 (its
+// quads {W^jm, W^2jm, W^3jm}), loaded one stage ahead so that their latency hides behind the
+// stage before.  A butterfly's j (and so its slot here) does not depend on the stage.
 template <typename T2, int EPT, int NT>
+__device__ __forceinline__ void stage4_twiddles(T2 (&w)[EPT / 4][3], int r, int log2ncol,
+                                                int log2ns, const T2* __restrict__ tw4) {
+  const int Rq = 1 << (r - 2);
+  const int nbf = (1 << log2ncol) * Rq;
+  const int ns = 1 << log2ns;
+  const T2* __restrict__ q4 = tw4 + 4 * ((ns - 4) / 3);   // this stage's quads
+#pragma unroll
+  for (int kk = 0; kk < EPT / 4; ++kk) {
+    const int bi = min((int)threadIdx.x + kk * NT, nbf - 1);
+    const T2* q = q4 + 4 * ((bi & (Rq - 1)) & (ns - 1));
+    w[kk][0] = q[0];
+    w[kk][1] = q[1];
+    w[kk][2] = q[2];
+  }
+}
+
+template <typename T2, int EPT, int NT, bool TWID>
 __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol, int log2ns,
-                                           const T2* __restrict__ tw4, int inverse) {
+                                           const T2 (&w)[EPT / 4][3], int inverse) {
   // Butterfly indices past the tile (partial tiles) are clamped to the last butterfly, whose
-  // thread stores the same values: no per-butterfly branches, so every LDS read and twiddle
-  // load of the thread is in flight together.  Twiddled (ns > 1) and plain stages are separate
-  // straight-line copies.
+  // thread stores the same values: no per-butterfly branches, so every LDS read of the thread
+  // is in flight together.  TWID: ns > 1 (twiddles w from stage4_twiddles).
   const int tid = threadIdx.x;
   const int Rq = 1 << (r - 2);
   const int nbf = (1 << log2ncol) * Rq;
   const int ns = 1 << log2ns;
   T2 v[EPT];
-  if (ns > 1) {
-    const T2* __restrict__ q4 = tw4 + 4 * ((ns - 4) / 3);   // this stage's quads
 #pragma unroll
-    for (int kk = 0; kk < EPT / 4; ++kk) {
-      const int bi = min(tid + kk * NT, nbf - 1);
-      const int c = bi >> (r - 2);
-      const int j = bi & (Rq - 1);
-      const T2* col = lds + c * RP;
-      T2 a0 = col[LPAD(j)], a1 = col[LPAD(j + Rq)], a2 = col[LPAD(j + 2 * Rq)],
-         a3 = col[LPAD(j + 3 * Rq)];
-      const T2* w = q4 + 4 * (j & (ns - 1));   // {W^jm, W^2jm, W^3jm, 0}
-      a1 = cmul(a1, w[0]);
-      a2 = cmul(a2, w[1]);
-      a3 = cmul(a3, w[2]);
-      bfly4(a0, a1, a2, a3, inverse);
-      v[4 * kk + 0] = a0;
-      v[4 * kk + 1] = a1;
-      v[4 * kk + 2] = a2;
-      v[4 * kk + 3] = a3;
+  for (int kk = 0; kk < EPT / 4; ++kk) {
+    const int bi = min(tid + kk * NT, nbf - 1);
+    const int c = bi >> (r - 2);
+    const int j = bi & (Rq - 1);
+    const T2* col = lds + c * RP;
+    T2 a0 = col[LPAD(j)], a1 = col[LPAD(j + Rq)], a2 = col[LPAD(j + 2 * Rq)],
+       a3 = col[LPAD(j + 3 * Rq)];
+    if (TWID) {
+      a1 = cmul(a1, w[kk][0]);
+      a2 = cmul(a2, w[kk][1]);
+      a3 = cmul(a3, w[kk][2]);
     }
-  } else {
-#pragma unroll
-    for (int kk = 0; kk < EPT / 4; ++kk) {
-      const int bi = min(tid + kk * NT, nbf - 1);
-      const int c = bi >> (r - 2);
-      const int j = bi & (Rq - 1);
-      const T2* col = lds + c * RP;
-      T2 a0 = col[LPAD(j)], a1 = col[LPAD(j + Rq)], a2 = col[LPAD(j + 2 * Rq)],
-         a3 = col[LPAD(j + 3 * Rq)];
-      bfly4(a0, a1, a2, a3, inverse);
-      v[4 * kk + 0] = a0;
-      v[4 * kk + 1] = a1;
-      v[4 * kk + 2] = a2;
-      v[4 * kk + 3] = a3;
-    }
+    bfly4(a0, a1, a2, a3, inverse);
+    v[4 * kk + 0] = a0;
+    v[4 * kk + 1] = a1;
+    v[4 * kk + 2] = a2;
+    v[4 * kk + 3] = a3;
   }
   __syncthreads();
 #pragma unroll
@@ -418,10 +420,28 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
   // ---- sub-FFT of every column: radix-4 stages, then one radix-2 stage if r is odd
   const T2* __restrict__ twR = (const T2*)A.twR;
   const T2* __restrict__ tw4 = (const T2*)A.tw4;
+  // The twiddles of radix-4 stage log2ns + 2 are loaded during stage log2ns (at the last one,
+  // the last stage's again: an unconditional load, so no wait is added for it).
+  const int l2last = r - 2 - (r & 1);   // log2ns of the last radix-4 stage
+  T2 wc[EPT / 4][3];
   int log2ns = 0;
   int rem = r;
+  if (rem >= 2) {
+    if (l2last >= 2) stage4_twiddles<T2, EPT, NT>(wc, r, log2ncol, 2, tw4);
+    fft_stage4<T2, EPT, NT, false>(lds, RP, r, log2ncol, 0, wc, A.inverse);
+    log2ns = 2;
+    rem -= 2;
+  }
   while (rem >= 2) {
-    fft_stage4<T2, EPT, NT>(lds, RP, r, log2ncol, log2ns, tw4, A.inverse);
+    T2 wn[EPT / 4][3];
+    stage4_twiddles<T2, EPT, NT>(wn, r, log2ncol, min(log2ns + 2, l2last), tw4);
+    fft_stage4<T2, EPT, NT, true>(lds, RP, r, log2ncol, log2ns, wc, A.inverse);
+#pragma unroll
+    for (int kk = 0; kk < EPT / 4; ++kk) {
+      wc[kk][0] = wn[kk][0];
+      wc[kk][1] = wn[kk][1];
+      wc[kk][2] = wn[kk][2];
+    }
     log2ns += 2;
     rem -= 2;
   }

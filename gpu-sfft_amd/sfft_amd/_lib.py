@@ -73,7 +73,8 @@ EXPORTED_FUNCTIONS = [
     "sfft_plan_workspace_bytes", "sfft_plan_set_filter", "sfft_execute_async", "sfft_execute",
     "sfft_plan_set_profiling", "sfft_plan_set_graph", "sfft_plan_stage_times",
     "sfft_plan_interval_times",
-    "sfft_plan_debug_buffer", "sfft_plan_comb_key_order", "sfft_value_apply", "sfft_value_adjoint",
+    "sfft_plan_debug_buffer", "sfft_plan_comb_key_order", "sfft_plan_front_end",
+    "sfft_value_apply", "sfft_value_adjoint",
     "sfft_batch_create", "sfft_batch_destroy", "sfft_batch_plan", "sfft_batch_set_filter",
     "sfft_batch_set_graph", "sfft_batch_execute_async", "sfft_batch_execute",
     "sfft_batch_counts_device", "sfft_batch_tune", "sfft_batch_set_shape",
@@ -133,6 +134,8 @@ def _declare(L):
     L.sfft_plan_debug_buffer.restype = _p
     L.sfft_plan_comb_key_order.argtypes = [_p, C.POINTER(_i), C.POINTER(_i)]
     L.sfft_plan_comb_key_order.restype = _i
+    L.sfft_plan_front_end.argtypes = [_p]
+    L.sfft_plan_front_end.restype = _i
     L.sfft_value_apply.argtypes = [_p, _p, C.POINTER(Draws), _i, _p, _p, _p]
     L.sfft_value_apply.restype = _i
     L.sfft_value_adjoint.argtypes = [_p, _p, C.POINTER(Draws), _i, _p, _p, _p]

@@ -271,6 +271,11 @@ class Plan:
         torch.cuda.synchronize()
         return gx
 
+    @property
+    def front_end(self):
+        """The front end this plan runs ("gather", "stream" or "columns"; AUTO resolved)."""
+        return {v: k for k, v in FRONT.items()}[lib.sfft_plan_front_end(self._h)]
+
     def comb_key_order(self):
         """(lo, mid) of the comb keys' storage order (debug buffer 8): position s holds bin
         (s mod 2^lo) | ((s >> lo) mod 2^mid) << (log2 W - mid) | (s >> (lo + mid)) << lo."""

@@ -169,8 +169,9 @@ typedef struct {
  * B columns, so the fold walks the columns in blocks of one 128-byte line width for every loop
  * at once and the taps of different loops that fall in the same line share its fetch; the
  * filter taps are read from a bucket-major copy built when the filter is set (B*(w/B+1)
- * entries in the plan workspace).  AUTO picks COLUMNS for FP32 plans (where the fold streams
- * the 256-byte-wide column slabs through LDS) and GATHER for FP64 plans. */
+ * entries in the plan workspace).  AUTO picks COLUMNS wherever the fold can stream the
+ * 256-byte-wide column slabs through LDS (both precisions at n = 2^27, k = 16000), else GATHER;
+ * sfft_plan_front_end reports the choice. */
 typedef enum {
   SFFT_FRONT_AUTO = 0,
   SFFT_FRONT_GATHER = 1,
@@ -245,6 +246,9 @@ const void *sfft_plan_debug_buffer(const sfft_plan *plan, int which);
  * comb loop's keys holds bin (s mod 2^lo) | ((s >> lo) mod 2^mid) << (log2 W_RLFC - mid) |
  * (s >> (lo + mid)) << lo; mid = 0 means natural order. */
 sfft_status sfft_plan_comb_key_order(const sfft_plan *plan, int *lo, int *mid);
+/* No counterpart in the reference (an inspection aid): the front end the plan runs, with
+ * SFFT_FRONT_AUTO resolved (SFFT_FRONT_GATHER, _STREAM or _COLUMNS; -1 for a NULL plan). */
+int sfft_plan_front_end(const sfft_plan *plan);
 
 /* ---- the value map and its adjoint (FP64 plans) ----------------------------------------
  * No counterpart in the reference: the gradient of the estimated values (innerLoop.cu:

@@ -7,7 +7,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import N20_K1000, SMALL_CONFIGS, bits, draws, oracle_filter, params, sparse_signal
+from helpers import (N20_K1000, N27_K16000, SMALL_CONFIGS, bits, draws, oracle_filter, params,
+                     sparse_signal)
 
 pytestmark = pytest.mark.gpu
 
@@ -124,6 +125,18 @@ def test_transform_front_ends_bitexact(S, cfg, fold_mode, front_end):
 # earlier loops; total loop counts 1, 5, 8, 10, 16 (the estimation kernel's compile-time loop
 # counts) and 17, 32 (its run-time loop count, up to the largest the plan accepts)
 LOOP_VARIANTS = [(1, 1, 0), (3, 1, 2), (4, 3, 4), (3, 3, 7), (5, 4, 11), (6, 4, 11), (8, 5, 24)]
+
+
+def test_plan_front_end_reported(S):
+    """sfft_plan_front_end: explicit front ends come back unchanged; AUTO resolves to the
+    column front end at n = 2^27, k = 16000 in both precisions (its slab fold applies there)."""
+    prm = params(SMALL_CONFIGS[1])
+    for fe in ("gather", "stream", "columns"):
+        assert S.Plan(prm, "fp64", front_end=fe).front_end == fe
+    assert S.Plan(prm, "fp64").front_end in ("gather", "columns")
+    big = params(N27_K16000)
+    for prec in ("fp64", "fp32"):
+        assert S.Plan(big, prec).front_end == "columns"
 
 
 @pytest.mark.parametrize("loops", LOOP_VARIANTS, ids=lambda v: "loc%d_thr%d_est%d" % v)
