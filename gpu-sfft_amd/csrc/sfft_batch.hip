@@ -7,10 +7,13 @@
 //   * every signal's other stage groups (clear + comb restriction, then bucket FFT + cutoff,
 //     reverse hash, estimation) on one of several worker streams, joined back to the batch
 //     stream;
-// either enqueued eagerly or captured once into a hipGraph and replayed (the draws are
-// uploaded before the replay, outside the graph; the graph holds no memset nodes: buffers
-// are cleared by kernels).  A graph is re-captured when the signal pointers or the tuned
-// shapes change.
+// either enqueued eagerly or replayed from hipGraphs: each worker stream's front groups and
+// its back groups are captured once as two linear chains (on that stream), and a replay
+// launches them around the directly launched permute+filter (one graph over all streams
+// replayed slower than the eager launches: its cross-stream dependencies serialise the
+// ~1,600 nodes).  The draws are uploaded before the replay, outside the graphs; the graphs
+// hold no memset nodes (buffers are cleared by kernels).  The graphs are re-captured when
+// the signal pointers or the tuned shapes change.
 //
 // Autotuner: times, on the batch itself, only launch shapes that compute the same bits --
 // the permute+filter loops per thread, block size and persistent grid (every shape runs the
@@ -44,18 +47,24 @@ struct sfft_batch {
   // per-signal hit counts, contiguous
   int* d_counts = nullptr;
   int* h_counts = nullptr;
-  // graph
+  // graphs: per worker stream, one of its signals' front groups and one of their back groups
+  // (linear chains, each captured on its own worker stream); the persistent permute+filter
+  // between them is launched directly
   int use_graph = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  std::vector<hipGraph_t> graphs;
+  std::vector<hipGraphExec_t> execs;   // [2*s] front, [2*s+1] back of worker s (may be null)
+  bool captured = false;
   std::vector<const void*> graph_x;
 };
 
 static void drop_graph(sfft_batch* b) {
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
+  for (auto e : b->execs)
+    if (e) (void)hipGraphExecDestroy(e);
+  for (auto g : b->graphs)
+    if (g) (void)hipGraphDestroy(g);
+  b->execs.clear();
+  b->graphs.clear();
+  b->captured = false;
   b->graph_x.clear();
 }
 
@@ -83,7 +92,9 @@ extern "C" sfft_status sfft_batch_create(const sfft_params* params, int count,
   for (int i = 0; i < count; ++i) {
     const sfft_status s = sfft_params_validate(&params[i]);
     if (s != SFFT_OK) return s;
-    if (params[i].front_end == SFFT_FRONT_STREAM) return SFFT_ERR_INVALID;
+    // the batch folds every signal with the gathering permute+filter (one persistent launch)
+    if (params[i].front_end != SFFT_FRONT_AUTO && params[i].front_end != SFFT_FRONT_GATHER)
+      return SFFT_ERR_INVALID;
   }
   sfft_batch* b = new sfft_batch();
   b->count = count;
@@ -98,7 +109,9 @@ extern "C" sfft_status sfft_batch_create(const sfft_params* params, int count,
   }
   for (int i = 0; i < count; ++i) {
     sfft_plan* p = nullptr;
-    const sfft_status s = sfft_plan_create(&params[i], prec, b->st, &p);
+    sfft_params pi = params[i];
+    pi.front_end = SFFT_FRONT_GATHER;   // AUTO: the batch's own front end
+    const sfft_status s = sfft_plan_create(&pi, prec, b->st, &p);
     if (s != SFFT_OK) {
       sfft_batch_destroy(b);
       return s;
@@ -162,31 +175,78 @@ static sfft_status build_jobs(sfft_batch* b, const void* const* d_x) {
   return SFFT_OK;
 }
 
-// all stage launches of one batch execution, from the batch stream (fork/join)
-static sfft_status enqueue(sfft_batch* b, const void* const* d_x) {
+// the stage groups of worker s's signals (i = s, s + kWorkers, ...): the front groups (clear,
+// comb FFT, comb cutoff, bitmaps) or the back groups (bucket FFT, cutoff, reverse hash,
+// estimation; after the fold)
+static sfft_status enqueue_worker(sfft_batch* b, const void* const* d_x, int s, bool back) {
+  for (int i = s; i < b->count; i += kWorkers) {
+    sfft_plan* P = b->plans[i];
+    const HashArgs h = hash_args(P, b->d_counts + i);
+    const sfft_status v = back ? run_back(P, h, nullptr, b->workers[s], false)
+                               : run_front(P, d_x[i], h, b->workers[s], false);
+    if (v != SFFT_OK) return v;
+  }
+  return SFFT_OK;
+}
+
+// one batch execution from the batch stream (fork/join): the persistent permute+filter of
+// all signals and the workers' front groups run concurrently, then the back groups.  Eager
+// launches go round-robin over the signals (so every worker starts at once); with graphs,
+// each worker's groups are replayed from its two captured chains.
+static sfft_status enqueue(sfft_batch* b, const void* const* d_x, bool graphs) {
   HIPCK(hipEventRecord(b->ev_fork, b->st));
   HIPCK(perm_filter_batch_launch(b->fp32, b->d_jobs, b->count, b->total_items, b->lpt, b->nt,
                                  b->grid, b->st));
   HIPCK(hipEventRecord(b->ev_pf, b->st));
   for (int s = 0; s < kWorkers; ++s) HIPCK(hipStreamWaitEvent(b->workers[s], b->ev_fork, 0));
-  // front groups first on every worker, then the back groups (they wait for the fold)
-  for (int i = 0; i < b->count; ++i) {
-    sfft_plan* P = b->plans[i];
-    const HashArgs h = hash_args(P, b->d_counts + i);
-    const sfft_status v = run_front(P, d_x[i], h, b->workers[i % kWorkers], false);
-    if (v != SFFT_OK) return v;
-  }
-  for (int s = 0; s < kWorkers; ++s) HIPCK(hipStreamWaitEvent(b->workers[s], b->ev_pf, 0));
-  for (int i = 0; i < b->count; ++i) {
-    sfft_plan* P = b->plans[i];
-    const HashArgs h = hash_args(P, b->d_counts + i);
-    const sfft_status v = run_back(P, h, nullptr, b->workers[i % kWorkers], false);
-    if (v != SFFT_OK) return v;
+  for (int back = 0; back < 2; ++back) {
+    if (back)
+      for (int s = 0; s < kWorkers; ++s) HIPCK(hipStreamWaitEvent(b->workers[s], b->ev_pf, 0));
+    if (graphs) {
+      for (int s = 0; s < kWorkers; ++s)
+        if (b->execs[2 * s + back]) HIPCK(hipGraphLaunch(b->execs[2 * s + back], b->workers[s]));
+      continue;
+    }
+    for (int i = 0; i < b->count; ++i) {
+      sfft_plan* P = b->plans[i];
+      const HashArgs h = hash_args(P, b->d_counts + i);
+      hipStream_t w = b->workers[i % kWorkers];
+      const sfft_status v = back ? run_back(P, h, nullptr, w, false)
+                                 : run_front(P, d_x[i], h, w, false);
+      if (v != SFFT_OK) return v;
+    }
   }
   for (int s = 0; s < kWorkers; ++s) {
     HIPCK(hipEventRecord(b->ev_join[s], b->workers[s]));
     HIPCK(hipStreamWaitEvent(b->st, b->ev_join[s], 0));
   }
+  return SFFT_OK;
+}
+
+// capture every worker's front and back chains (each on its own worker stream)
+static sfft_status capture(sfft_batch* b, const void* const* d_x) {
+  b->graphs.assign(2 * kWorkers, nullptr);
+  b->execs.assign(2 * kWorkers, nullptr);
+  for (int s = 0; s < kWorkers && s < b->count; ++s) {
+    for (int part = 0; part < 2; ++part) {
+      HIPCK(hipStreamBeginCapture(b->workers[s], hipStreamCaptureModeThreadLocal));
+      const sfft_status v = enqueue_worker(b, d_x, s, part == 1);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->workers[s], &g);
+      if (v != SFFT_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        drop_graph(b);
+        return v != SFFT_OK ? v : SFFT_ERR_HIP;
+      }
+      b->graphs[2 * s + part] = g;
+      if (hipGraphInstantiate(&b->execs[2 * s + part], g, nullptr, nullptr, 0) != hipSuccess) {
+        drop_graph(b);
+        return SFFT_ERR_HIP;
+      }
+    }
+  }
+  b->captured = true;
+  b->graph_x.assign(d_x, d_x + b->count);
   return SFFT_OK;
 }
 
@@ -212,25 +272,14 @@ extern "C" sfft_status sfft_batch_execute_async(sfft_batch* b, const void* const
   if (!b || !d_x || !draws) return SFFT_ERR_INVALID;
   sfft_status v = prepare(b, d_x, draws);
   if (v != SFFT_OK) return v;
-  if (!b->use_graph) return enqueue(b, d_x);
-  if (!b->exec) {
-    HIPCK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-    v = enqueue(b, d_x);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(b->st, &g);
-    if (v != SFFT_OK || e != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return v != SFFT_OK ? v : SFFT_ERR_HIP;
-    }
-    b->graph = g;
-    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
-      drop_graph(b);
-      return SFFT_ERR_HIP;
-    }
-    b->graph_x.assign(d_x, d_x + b->count);
+  if (!b->use_graph) return enqueue(b, d_x, false);
+  if (!b->captured) {
+    // nothing in flight may still use the worker streams' captured work
+    for (int s = 0; s < kWorkers; ++s) HIPCK(hipStreamSynchronize(b->workers[s]));
+    v = capture(b, d_x);
+    if (v != SFFT_OK) return v;
   }
-  HIPCK(hipGraphLaunch(b->exec, b->st));
-  return SFFT_OK;
+  return enqueue(b, d_x, true);
 }
 
 extern "C" sfft_status sfft_batch_execute(sfft_batch* b, const void* const* d_x,

@@ -327,11 +327,13 @@ class Batch:
     permute+filter launch for all signals, the other stages on worker streams, optional
     hipGraph capture and an autotuner over bit-equivalent launch shapes."""
 
-    def __init__(self, params_list, precision="fp32", fold_mode=FOLD_REFERENCE):
+    def __init__(self, params_list, precision="fp32", fold_mode=FOLD_REFERENCE,
+                 front_end="gather"):
+        """front_end: "gather" or "auto" (the batch always gathers); others are refused."""
         self.params = [dict(p) for p in params_list]
         self.count = len(self.params)
         self.fp32 = precision == "fp32"
-        arr = (L.Params * self.count)(*[_cparams(p, fold_mode, 0, "gather")
+        arr = (L.Params * self.count)(*[_cparams(p, fold_mode, 0, front_end)
                                        for p in self.params])
         h = C.c_void_p()
         _check(lib.sfft_batch_create(arr, self.count, L.SFFT_FP32 if self.fp32 else L.SFFT_FP64,

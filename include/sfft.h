@@ -273,11 +273,13 @@ sfft_status sfft_value_adjoint(sfft_plan *plan, const complex_t *d_grad, const s
 /* ---- batches of uneven signals -----------------------------------------------------------
  * No counterpart in the reference (which transforms one signal per outer_loop call): count
  * signals, each with its own parameters (n, k, B, ... may all differ), transformed together.
- * The permute+filter of all signals runs as one persistent launch; every signal's other
- * stages run on worker streams; with sfft_batch_set_graph(b, 1) the whole launch sequence is
- * captured into a hipGraph once and replayed (re-captured when the signal pointers, filters
- * or shapes change).  Each signal's results are bit-identical to sfft_execute of the same
- * signal with the same plan parameters and draws.  stream NULL: the batch makes its own. */
+ * The permute+filter of all signals runs as one persistent launch (the gathering front end:
+ * params[i].front_end must be SFFT_FRONT_AUTO or SFFT_FRONT_GATHER); every signal's other
+ * stages run on worker streams; with sfft_batch_set_graph(b, 1) each worker stream's launch
+ * sequence is captured into hipGraphs once and replayed (re-captured when the signal
+ * pointers, filters or shapes change).  Each signal's results are bit-identical to
+ * sfft_execute of the same signal with the same plan parameters and draws.  stream NULL:
+ * the batch makes its own. */
 typedef struct sfft_batch sfft_batch;
 sfft_status sfft_batch_create(const sfft_params *params, int count, sfft_precision prec,
                               void *stream, sfft_batch **out);

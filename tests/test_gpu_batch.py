@@ -57,6 +57,20 @@ def _same(got, want):
         assert gn.tobytes() == wn.tobytes(), f"signal {i}"
 
 
+def test_batch_front_end_param(S):
+    """The batch gathers: front_end AUTO is accepted (and computes the same bits), COLUMNS and
+    STREAM are refused by sfft_batch_create."""
+    prms, xs, fts, wins, draws = _setup(S, "fp32", count=3)
+    b = S.Batch(prms, "fp32", front_end="auto")
+    for i in range(len(prms)):
+        b.set_filter(i, fts[i], wins[i])
+    _same(b.execute(xs, draws), _singles(S, prms, xs, fts, wins, draws, "fp32"))
+    b.close()
+    for fe in ("columns", "stream"):
+        with pytest.raises(S.SfftError):
+            S.Batch(prms, "fp32", front_end=fe)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_batch_equals_single_transforms(S, precision):
     import torch
