@@ -96,8 +96,10 @@ constexpr int FFT_QMAX = 16;                 // columns per tile
 template <typename Real, int LOAD = 0>
 struct Tile {   // LOAD: kLoadComb (gathers from x); kLoadPlain, kLoadCombRows, kLoadPerm (plain)
   static constexpr int cap = LOAD == 1 ? 2048 : (sizeof(Real) == 8 ? 4096 : 8192);
-  // threads per workgroup
-  static constexpr int nt = LOAD == 1 ? (sizeof(Real) == 8 ? 512 : 256) : 1024;
+  // threads per workgroup; FP32 plain tiles: 512, so that a radix-16 step's 16 complex64
+  // values and their twiddles fit the 128 registers of two 512-thread workgroups per CU
+  // (FP64 keeps 1024 threads and single radix-4 stages: measured faster at 2^22)
+  static constexpr int nt = LOAD == 1 ? (sizeof(Real) == 8 ? 512 : 256) : (sizeof(Real) == 8 ? 1024 : 512);
   static constexpr int ept = cap / nt;                 // elements per thread
 };
 static int tile_cap(int is_fp32, int load) {
@@ -217,6 +219,64 @@ __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol,
   __syncthreads();
 }
 
+// Two radix-4 stages (spans ns and 4ns) in registers: task (column c, tix = a0*ns + jm), one
+// per thread, reads the 16 elements tix + u*R/16 (u = t + 4m: input m of the first stage's
+// butterfly t, whose index is tix + t*R/16), runs the four first-stage butterflies (twiddles
+// W_{4ns}^{m*jm}, the same for every t), then the four second-stage butterflies (butterfly m
+// takes the first stage's outputs m of t = 0..3; twiddles W_{16ns}^{t*(m*ns + jm)}) and writes
+// output m'' of butterfly m to a0*16ns + jm + (m + 4m'')*ns.  These are the two stages' own
+// operations in their own order (the same bits), with one LDS round trip instead of two.
+template <typename T2, int NT, bool TW1>
+__device__ __forceinline__ void fft_stage16(T2* lds, int RP, int r, int log2ncol, int log2ns,
+                                            const T2* __restrict__ tw4, int inverse) {
+  const int tid = threadIdx.x;
+  const int R16 = 1 << (r - 4);
+  const int ntask = (1 << log2ncol) * R16;
+  const int ns = 1 << log2ns;
+  T2 x[16];
+  int base = 0, c = 0;
+  if (tid < ntask) {
+    c = tid >> (r - 4);
+    const int tix = tid & (R16 - 1);
+    const int jm = tix & (ns - 1);
+    base = ((tix >> log2ns) << (log2ns + 4)) + jm;
+    const T2* col = lds + c * RP;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) x[u] = col[LPAD(tix + u * R16)];
+    if (TW1) {
+      const T2* w = tw4 + 4 * ((ns - 4) / 3 + jm);
+      const T2 w1 = w[0], w2 = w[1], w3 = w[2];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        x[t + 4] = cmul(x[t + 4], w1);
+        x[t + 8] = cmul(x[t + 8], w2);
+        x[t + 12] = cmul(x[t + 12], w3);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) bfly4(x[t], x[t + 4], x[t + 8], x[t + 12], inverse);
+    // second stage: butterfly m takes x[4m' ... ] -> inputs m' = t: x[t + 4m]
+    const T2* q2 = tw4 + 4 * ((4 * ns - 4) / 3 + jm);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const T2* w = q2 + 4 * m * ns;
+      x[1 + 4 * m] = cmul(x[1 + 4 * m], w[0]);
+      x[2 + 4 * m] = cmul(x[2 + 4 * m], w[1]);
+      x[3 + 4 * m] = cmul(x[3 + 4 * m], w[2]);
+      bfly4(x[4 * m], x[1 + 4 * m], x[2 + 4 * m], x[3 + 4 * m], inverse);
+    }
+  }
+  __syncthreads();
+  if (tid < ntask) {
+    T2* col = lds + c * RP;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int m2 = 0; m2 < 4; ++m2) col[LPAD(base + (m + 4 * m2) * ns)] = x[m2 + 4 * m];
+  }
+  __syncthreads();
+}
+
 template <typename T2, int EPT, int NT>
 __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol, int log2ns,
                                            const T2* __restrict__ twR) {
@@ -270,8 +330,9 @@ __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol,
 enum PassTw { kTwNone = 0, kTwDirect = 1, kTwSplit = 2 };
 
 template <typename Real, int LOAD, int STORE, int TW>
-// 8 waves per SIMD for the 1024-thread tiles: two workgroups per CU
-__global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void fft_pass_kernel(
+// two workgroups per CU for the plain tiles: 8 waves per SIMD at 1024 threads, 4 at 512
+__global__ __launch_bounds__((Tile<Real, LOAD>::nt),
+                             (LOAD == 1 ? 1 : Tile<Real, LOAD>::nt / 128)) void fft_pass_kernel(
     PassArgs A) {
   using T2 = typename CT<Real>::T2;
   using KT = typename CT<Real>::K;
@@ -327,9 +388,9 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
   const T2* __restrict__ cx =
       (LOAD == kLoadComb) ? (const T2*)A.comb_x + A.comb_offs[b] : (const T2*)nullptr;
   const uint32_t pm = (LOAD == kLoadPerm) ? (uint32_t)A.perm[b] : 0u;
-  // loads in flight per batch: the plain tiles take batches of 8 bytes x 4 (FP32) or
-  // 16 bytes x 2 (FP64), which keeps them within 64 registers (two 1024-thread workgroups
-  // per CU); the comb-gathering pass issues all of its loads together
+  // loads in flight per batch: the plain tiles take batches of 8 bytes x 4 (FP32, 512
+  // threads) or 16 bytes x 2 (FP64, 1024 threads: within 64 registers); the comb-gathering
+  // pass issues all of its loads together
   constexpr int LB = (LOAD == kLoadComb) ? EPT : (sizeof(Real) == 8 ? 2 : 4);
   constexpr bool split = (TW == kTwSplit);   // log2Lcur > kMaxPassLog
 #pragma unroll
@@ -420,27 +481,25 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt), (LOAD == 1 ? 1 : 8)) void f
   // ---- sub-FFT of every column: radix-4 stages, then one radix-2 stage if r is odd
   const T2* __restrict__ twR = (const T2*)A.twR;
   const T2* __restrict__ tw4 = (const T2*)A.tw4;
-  // The twiddles of radix-4 stage log2ns + 2 are loaded during stage log2ns (at the last one,
-  // the last stage's again: an unconditional load, so no wait is added for it).
-  const int l2last = r - 2 - (r & 1);   // log2ns of the last radix-4 stage
-  T2 wc[EPT / 4][3];
+  // FP32: pairs of radix-4 stages as radix-16 steps in registers, then a last radix-4 stage
+  // if two bits remain (the same stages in the same order); FP64: radix-4 stages
   int log2ns = 0;
   int rem = r;
-  if (rem >= 2) {
-    if (l2last >= 2) stage4_twiddles<T2, EPT, NT>(wc, r, log2ncol, 2, tw4);
-    fft_stage4<T2, EPT, NT, false>(lds, RP, r, log2ncol, 0, wc, A.inverse);
-    log2ns = 2;
-    rem -= 2;
+  while (sizeof(Real) == 4 && rem >= 4) {
+    if (log2ns == 0)
+      fft_stage16<T2, NT, false>(lds, RP, r, log2ncol, 0, tw4, A.inverse);
+    else
+      fft_stage16<T2, NT, true>(lds, RP, r, log2ncol, log2ns, tw4, A.inverse);
+    log2ns += 4;
+    rem -= 4;
   }
   while (rem >= 2) {
-    T2 wn[EPT / 4][3];
-    stage4_twiddles<T2, EPT, NT>(wn, r, log2ncol, min(log2ns + 2, l2last), tw4);
-    fft_stage4<T2, EPT, NT, true>(lds, RP, r, log2ncol, log2ns, wc, A.inverse);
-#pragma unroll
-    for (int kk = 0; kk < EPT / 4; ++kk) {
-      wc[kk][0] = wn[kk][0];
-      wc[kk][1] = wn[kk][1];
-      wc[kk][2] = wn[kk][2];
+    T2 wc[EPT / 4][3];
+    if (log2ns > 0) {
+      stage4_twiddles<T2, EPT, NT>(wc, r, log2ncol, log2ns, tw4);
+      fft_stage4<T2, EPT, NT, true>(lds, RP, r, log2ncol, log2ns, wc, A.inverse);
+    } else {
+      fft_stage4<T2, EPT, NT, false>(lds, RP, r, log2ncol, 0, wc, A.inverse);
     }
     log2ns += 2;
     rem -= 2;
