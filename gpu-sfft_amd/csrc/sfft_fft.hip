@@ -532,8 +532,7 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
         const KT key = sample_key(val);
         keys[kpos] = key;
         if (A.digits)
-          A.digits[b * A.keys_bstride + kpos] =
-              (uint16_t)((key >> (sizeof(KT) * 8 - 13)) & (KT)0xFFF);
+          A.digits[b * A.keys_bstride + kpos] = (uint16_t)sel_first_digit(key);
       }
     }
   }

@@ -40,6 +40,17 @@ struct FftTables {
 hipError_t fft_tables_create(int log2n, int inverse, int is_fp32, FftTables* t);
 void fft_tables_destroy(FftTables* t);
 
+// The select's first digit of a |X|^2 key (12 bits, a non-decreasing function of the key, so
+// that digit > d0 means key > every key of digit d0): FP32 keys, their top 12 bits below the
+// sign; FP64 keys, the top 12 bits below the sign of the value rounded to FP32 (8 exponent and
+// 4 mantissa bits -- the FP64 pattern's own top 12 bits are its 11 exponent bits and one
+// mantissa bit, which leaves a whole binade of candidates below the threshold digit).
+__device__ __forceinline__ unsigned sel_first_digit(unsigned key) { return (key >> 19) & 0xFFFu; }
+__device__ __forceinline__ unsigned sel_first_digit(unsigned long long key) {
+  const float f = __double2float_rn(__longlong_as_double((long long)key));
+  return (__float_as_uint(f) >> 19) & 0xFFFu;
+}
+
 enum FftLoad { kLoadPlain = 0, kLoadComb = 1, kLoadCombRows = 2, kLoadPerm = 3 };
 enum FftStore { kStoreCplx = 0, kStoreKeys = 1, kStoreBoth = 2 };
 

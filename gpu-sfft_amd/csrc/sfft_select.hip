@@ -40,10 +40,13 @@ struct KeyBits;
 template <>
 struct KeyBits<unsigned long long> {
   static constexpr int KB = 63;  // bits below the (always zero) sign bit
+  // the first digit is not the key's top bits (sel_first_digit): the refine starts at bit 0
+  static constexpr bool digit_is_prefix = false;
 };
 template <>
 struct KeyBits<unsigned int> {
   static constexpr int KB = 31;
+  static constexpr bool digit_is_prefix = true;
 };
 
 // bits of the index field of a composite key: enough for N - 1 (any N, not only powers of 2)
@@ -134,7 +137,7 @@ __device__ __forceinline__ int wave_append(int* counter, bool pred) {
 
 template <typename K>
 __device__ __forceinline__ unsigned top_digit(K key) {
-  return (unsigned)((key >> (KeyBits<K>::KB - SEL_DIG)) & (K)(SEL_BINS - 1));
+  return sel_first_digit(key);
 }
 
 template <typename K>
@@ -361,6 +364,28 @@ __global__ __launch_bounds__(CMP_NT) void sel_compact_kernel(int N, int Ksel, in
   }
 }
 
+// minimum of v over a 1024-thread block (every thread gets it); s: 16 entries of LDS
+__device__ __forceinline__ u128 block_min_u128(u128 v, u128* s) {
+  unsigned long long hi = (unsigned long long)(v >> 64), lo = (unsigned long long)v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long h2 = __shfl_xor(hi, o, 64), l2 = __shfl_xor(lo, o, 64);
+    if (h2 < hi || (h2 == hi && l2 < lo)) {
+      hi = h2;
+      lo = l2;
+    }
+  }
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s[w] = ((u128)hi << 64) | (u128)lo;
+  __syncthreads();
+  u128 m = s[0];
+  for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
+    if (s[i] < m) m = s[i];
+  __syncthreads();
+  return m;
+}
+
 // ---- 3. refine over the candidates (one workgroup per segment) ----------------------------
 // The first level scans the candidate list; from then on only the survivors of the previous
 // level (the composite keys under the chosen prefix, usually a handful) are scanned, from LDS
@@ -385,6 +410,7 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
   __shared__ int s_len[2];
   __shared__ int sh[16];
   __shared__ int s_digit, s_above, s_exit;
+  __shared__ u128 s_min[16];
   __shared__ int s_out;
   __shared__ int s_pre[SEL_MAX_CHUNKS + 1];
   __shared__ int s_bc[SEL_MAX_CHUNKS];
@@ -417,8 +443,10 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
   const int* ci = cidx + (long long)seg * N;
   if (tid == 0) s_out = 0;
   int r = st[seg].need0;
-  u128 prefix = (u128)(unsigned)st[seg].d0;   // the top SEL_DIG bits of c
-  int consumed = SEL_DIG;
+  // the candidates share the first digit d0; where it is the top SEL_DIG bits of c the radix
+  // continues below it, else it starts over at the top of c (one more level)
+  u128 prefix = KeyBits<K>::digit_is_prefix ? (u128)(unsigned)st[seg].d0 : (u128)0;
+  int consumed = KeyBits<K>::digit_is_prefix ? SEL_DIG : 0;
   u128 T = 0;
   bool done = false;
   int cur = -1;   // survivors: -1 = the candidate list, else lst[cur][0 .. s_len[cur])
@@ -475,6 +503,25 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
     consumed += width;
     if (s_exit) {
       T = prefix << shift;
+      if (!KeyBits<K>::digit_is_prefix && shift > 0) {
+        // keys outside the candidate list can share these top bits (their first digit is
+        // below d0, so they are smaller than every candidate): the threshold is the smallest
+        // candidate under the prefix instead of the prefix's lower bound
+        u128 mn = ~(u128)0;
+        if (cur < 0) {
+          for (int j = tid; j < nc; j += 1024) {
+            const long long q = at(j);
+            const u128 c = composite<K>(ck[q], (unsigned)ci[q], N, IB);
+            if ((c >> shift) == prefix && c < mn) mn = c;
+          }
+        } else {
+          for (int j = tid; j < s_len[cur]; j += 1024) {
+            const u128 c = lst[cur][j];
+            if ((c >> shift) == prefix && c < mn) mn = c;
+          }
+        }
+        T = block_min_u128(mn, s_min);
+      }
       done = true;
     } else {
       // keep the survivors: the keys whose bits above `shift` equal the new prefix
@@ -501,6 +548,19 @@ __global__ __launch_bounds__(1024) void sel_refine_kernel(int N, int Ksel, int e
       }
       __syncthreads();
       cur = (s_len[nxt] <= RF_CAP) ? nxt : -1;
+      if (cur >= 0 && s_len[cur] <= 1024) {
+        // few survivors: the r-th largest of them directly, by rank (composites are distinct)
+        const int S = s_len[cur];
+        if (tid < S) {
+          const u128 c = lst[cur][tid];
+          int above = 0;
+          for (int j = 0; j < S; ++j) above += lst[cur][j] > c ? 1 : 0;
+          if (above == r - 1) s_min[0] = c;
+        }
+        __syncthreads();
+        T = s_min[0];
+        done = true;
+      }
     }
     __syncthreads();
   }
