@@ -222,6 +222,10 @@ def kernel_roofline(prm, prec, front, iv_ms, config):
             entry = json.load(fh).get(f"{kname}_{config}_{prec}")
         if isinstance(entry, dict):
             traffic, tsrc = entry.get("bytes"), entry.get("source")
+            import hashlib
+            with open(os.path.join(ROOT, "gpu-sfft_amd", "csrc", "sfft_fft.hip"), "rb") as fh:
+                sha = hashlib.sha256(fh.read()).hexdigest()[:16]
+            extra["traffic_matches_source"] = entry.get("kernel_source_sha256") == sha
     out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
            "traffic_source": tsrc, "kernel": kname, "bytes_per_launch": alg,

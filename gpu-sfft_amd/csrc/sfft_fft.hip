@@ -9,6 +9,9 @@
 // explicitly (fma where fused, plain ops elsewhere); the library is built with
 // -ffp-contract=off so the compiler adds no fusion of its own.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "sfft_internal.h"
 
@@ -1285,16 +1288,26 @@ static hipError_t slab_launch(const void* x, const void* fT, void* bins, const i
   using T2 = typename CT<Real>::T2;
   const unsigned nslabs = 1u << (log2B - lsc);
   const size_t lds = slab_lds_bytes(sizeof(Real) == 4, loops, lsc, P);
-  // one workgroup per resident slot (each streams nslabs / grid slabs); any grid is correct
-  int dev = 0, cus = 0, per_cu = 0;
+  // one workgroup per resident slot (each streams nslabs / grid slabs); any grid is correct.
+  // The resident count is asked of the runtime once per (device, LDS size) and kept.
+  static std::mutex mu;
+  static std::map<std::pair<int, size_t>, int> resident;
+  int dev = 0;
   unsigned blocks = nslabs;
-  if (hipGetDevice(&dev) == hipSuccess &&
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void*>(&fold_slab_kernel<Real, PMAX, NT>), NT, lds) ==
-          hipSuccess &&
-      cus > 0 && per_cu > 0)
-    blocks = std::min(nslabs, (unsigned)(cus * per_cu));
+  if (hipGetDevice(&dev) == hipSuccess) {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = resident.find({dev, lds});
+    if (it == resident.end()) {
+      int cus = 0, per_cu = 0;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &per_cu, reinterpret_cast<const void*>(&fold_slab_kernel<Real, PMAX, NT>), NT,
+              lds) != hipSuccess)
+        cus = per_cu = 0;
+      it = resident.emplace(std::make_pair(dev, lds), cus * per_cu).first;
+    }
+    if (it->second > 0) blocks = std::min(nslabs, (unsigned)it->second);
+  }
   hipLaunchKernelGGL((fold_slab_kernel<Real, PMAX, NT>), dim3(blocks), dim3(NT), lds, st,
                      (const T2*)x, (const T2*)fT, (T2*)bins, a_dev, ai_dev, mask, log2B, log2m,
                      w, loops, tiled, T, P, lsc, (T2*)yc, offs, Rl, log2sigma);
