@@ -67,8 +67,9 @@ def parse():
                     help="both (default): the FP64 path (value) and the FP32 path")
     ap.add_argument("--front-end", default="auto",
                     choices=["auto", "columns", "gather", "stream"],
-                    help="signal access: auto (the plan's choice: column-ordered fold for "
-                         "FP32, gathers for FP64), columns, gathers in bucket order, or one "
+                    help="signal access: auto (the plan's choice: the column slab fold "
+                         "wherever its slab fits, which includes n=2^27 k=16000 in both "
+                         "precisions, else gathers), columns, gathers in bucket order, or one "
                          "streaming pass")
     ap.add_argument("--draws", type=int, default=8,
                     help="permutation / comb-offset draw sets the timed steps cycle through")
@@ -85,9 +86,10 @@ def parse():
                     help="run one plain device copy of this many MiB before the transforms "
                          "(a known-size kernel that calibrates HBM byte counters)")
     ap.add_argument("--gather-probe", action="store_true",
-                    help="run random-gather probes (torch index_select of 16-byte elements: "
-                         "one per random line, then two per line) for the counter granule "
-                         "evidence, and exit")
+                    help="run random-gather probes (torch index_select of 16- and 8-byte "
+                         "elements: one per random line, two per line in one 32-byte sector, "
+                         "two per line in different 64-byte halves) for the counter evidence "
+                         "of the HBM read granule, and exit")
     return ap.parse_args()
 
 
@@ -109,26 +111,43 @@ def host_info():
 
 
 def gather_probe():
-    """Known-size random gathers for the HBM granule: N 16-byte elements read from N random
-    128-byte lines of a 4 GiB array, then N elements read as N/2 pairs sharing a line.  Run it
-    under rocprofv3 --pmc FETCH_SIZE: bytes per gathered element = FETCH_SIZE / N."""
+    """Known-size random gathers for the HBM granule, one index_select dispatch per variant, in
+    this order: N 16-byte elements from N random 128-byte lines of a 4 GiB array (offset 0 in
+    the line, then a random 16-byte slot of it); N 8-byte elements from N random lines; N
+    16-byte elements as N/2 pairs in the same 32 bytes of a line; N/2 pairs split across the
+    line's two 64-byte halves (bytes 0-15 and 64-79).  Each dispatch also reads its N 8-byte
+    indices once (N/16 lines) and writes its output once.  Under rocprofv3 --pmc
+    TCC_EA0_RDREQ_sum: if a memory read request brings the whole 128-byte line, the split pairs
+    issue N/2 gather requests like the same-sector pairs; with 64-byte requests they issue N."""
     import torch
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     nel = 1 << 28                                   # complex128 elements, 4 GiB
     src = torch.zeros(nel, dtype=torch.complex128, device="cuda")
+    src8 = src.view(torch.complex64)                # the same 4 GiB as 8-byte elements
     N = 1 << 22
     g = torch.Generator(device="cuda").manual_seed(7)
     lines = torch.randperm(nel // 8, device="cuda", generator=g)[:N]
-    one = (lines * 8).contiguous()                                   # one element per line
-    pairs = (lines[: N // 2].repeat_interleave(2) * 8 +
-             torch.tensor([0, 1], device="cuda").repeat(N // 2)).contiguous()
-    torch.cuda.synchronize()
-    for name, idx, nl in (("one_per_line", one, N), ("two_per_line", pairs, N // 2)):
-        out = torch.index_select(src, 0, idx)
+    slot = torch.randint(0, 8, (N,), device="cuda", generator=g)
+    half = lines[: N // 2].repeat_interleave(2) * 8
+    variants = (
+        ("16B_one_per_line_offset0", src, (lines * 8), N),
+        ("16B_one_per_line_random_slot", src, (lines * 8 + slot), N),
+        ("8B_one_per_line", src8, (lines * 16), N),
+        ("16B_pairs_same_32B", src, (half + torch.tensor([0, 1], device="cuda").repeat(N // 2)),
+         N // 2),
+        ("16B_pairs_split_halves", src,
+         (half + torch.tensor([0, 4], device="cuda").repeat(N // 2)), N // 2),
+    )
+    for name, base, idx, nl in variants:
+        idx = idx.contiguous()
+        assert int(idx.min()) >= 0 and int(idx.max()) < base.numel()
         torch.cuda.synchronize()
-        log(f"[probe] {name}: {N} gathers of 16 B from {nl} lines, output "
-            f"{out.numel() * 16} bytes")
-    print(json.dumps({"probe": "random_gather", "elements": N, "element_bytes": 16}))
+        out = torch.index_select(base, 0, idx)
+        torch.cuda.synchronize()
+        log(f"[probe] {name}: {N} gathers of {base.element_size()} B from {nl} lines, "
+            f"output {out.numel() * out.element_size()} bytes")
+    print(json.dumps({"probe": "random_gather", "elements": N,
+                      "variants": [v[0] for v in variants]}))
 
 
 def run_precision(args, S, prm, prec, seeds_draws, snr, log):

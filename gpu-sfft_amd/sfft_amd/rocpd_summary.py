@@ -7,8 +7,16 @@ writes), used for the kernel timings and hardware counters kept under profiles/.
 `kernels`: per kernel name, the number of dispatches, mean / min / max duration (us) and the
 share of the summed kernel time (from the `kernels` view: end - start in ns).
 `counters`: per kernel name and counter, the number of dispatches and the mean value per
-dispatch (from the `counters_collection` view of a `--pmc` run).
+dispatch (from the `counters_collection` view of a `--pmc` run); with `--each` as the first
+NAME_PART, every dispatch's value instead, in dispatch order.
 NAME_PART filters the kernel names (substring match, any of them).
+
+    python3 gpu-sfft_amd/sfft_amd/rocpd_summary.py timeline RUN_results.db [LAST]
+
+`timeline`: per transform of a `--kernel-trace` run (from a `hash_clear_kernel` dispatch, the
+first kernel of a transform, to the next `estimate_kernel`, its last), the span, the union of
+kernel busy time and the idle rest; then the last LAST transforms of each precision (default 1)
+kernel by kernel: start, duration, grid, workgroup, LDS bytes and VGPRs per dispatch.
 
     python3 gpu-sfft_amd/sfft_amd/rocpd_summary.py join KERNELS.txt COUNTERS.txt [...]
 
@@ -68,6 +76,17 @@ def counters(path, parts):
     kn = next(c for c in ("kernel_name", "name") if c in ix)
     cn = next(c for c in ("counter_name", "name_counter", "counter") if c in ix)
     vn = next(c for c in ("value", "counter_value") if c in ix)
+    if parts and parts[0] == "--each":
+        parts = parts[1:]
+        dn = next((c for c in ("dispatch_id", "correlation_id", "id") if c in ix), None)
+        rows = [r for r in cur.execute(f"select * from {view}") if _keep(r[ix[kn]], parts)]
+        if dn:
+            rows.sort(key=lambda r: r[ix[dn]])
+        print(f"{'dispatch':>10} {'value':>18}  counter  kernel")
+        for r in rows:
+            d = r[ix[dn]] if dn else "-"
+            print(f"{d:>10} {float(r[ix[vn]]):18.1f}  {r[ix[cn]]}  {_short(r[ix[kn]])}")
+        return
     agg = collections.defaultdict(list)
     for r in cur.execute(f"select * from {view}"):
         if _keep(r[ix[kn]], parts):
@@ -75,6 +94,61 @@ def counters(path, parts):
     print(f"{'dispatches':>10} {'mean':>18}  counter  kernel")
     for (k, c), v in sorted(agg.items()):
         print(f"{len(v):10d} {sum(v) / len(v):18.1f}  {c}  {k}")
+
+
+def _transforms(rows):
+    """(first, last) row indices of each transform: hash_clear_kernel .. estimate_kernel."""
+    out, i = [], 0
+    while i < len(rows):
+        if "hash_clear_kernel" in rows[i]["name"]:
+            j = i
+            while j < len(rows) and "estimate_kernel" not in rows[j]["name"]:
+                j += 1
+            if j < len(rows):
+                out.append((i, j))
+            i = j + 1
+        else:
+            i += 1
+    return out
+
+
+def timeline(path, last):
+    cur = sqlite3.connect(path).cursor()
+    view = _view(cur, ("kernels",))
+    ix = _columns(cur, view)
+    keys = ("name", "start", "end", "grid_x", "grid_y", "workgroup_x", "lds_size", "vgpr_count")
+    rows = sorted(({k: r[ix[k]] for k in keys if k in ix} for r in cur.execute(
+        f"select * from {view}")), key=lambda d: d["start"])
+    spans = {}
+    print(f"{'precision':9s} {'span_us':>8s} {'busy_us':>8s} {'idle_us':>8s} {'kernels':>7s}")
+    for i, j in _transforms(rows):
+        s0, e0 = rows[i]["start"], rows[j]["end"]
+        busy, cs, ce = 0, None, None
+        for d in sorted(rows[i:j + 1], key=lambda d: d["start"]):
+            s, e = max(d["start"], s0), min(d["end"], e0)
+            if cs is None:
+                cs, ce = s, e
+            elif s <= ce:
+                ce = max(ce, e)
+            else:
+                busy, cs, ce = busy + ce - cs, s, e
+        busy += ce - cs
+        prec = "fp32" if any("<float" in d["name"] or "unsigned int>" in d["name"]
+                             for d in rows[i:j + 1]) else "fp64"
+        spans.setdefault(prec, []).append((i, j))
+        print(f"{prec:9s} {(e0 - s0) / 1e3:8.1f} {busy / 1e3:8.1f} {(e0 - s0 - busy) / 1e3:8.1f} "
+              f"{j - i + 1:7d}")
+    for prec, lst in spans.items():
+        for i, j in lst[-last:]:
+            s0 = rows[i]["start"]
+            print(f"\n{prec}: transform starting at dispatch {i}")
+            print(f"{'start_us':>8s} {'dur_us':>7s} {'grid':>12s} {'wg':>5s} {'lds_B':>6s} "
+                  f"{'vgpr':>4s}  kernel")
+            for d in rows[i:j + 1]:
+                grid = f"{d.get('grid_x', 0)}x{d.get('grid_y', 1)}"
+                print(f"{(d['start'] - s0) / 1e3:8.1f} {(d['end'] - d['start']) / 1e3:7.1f} "
+                      f"{grid:>12s} {d.get('workgroup_x', 0):5d} {d.get('lds_size', 0):6d} "
+                      f"{d.get('vgpr_count', 0):4d}  {_short(d['name']).replace('void ', '')}")
 
 
 def _read_table(path, cols):
@@ -113,6 +187,8 @@ def join(kernels_txt, counter_txts):
 def main(argv):
     if len(argv) >= 4 and argv[1] == "join":
         return join(argv[2], argv[3:])
+    if len(argv) >= 3 and argv[1] == "timeline":
+        return timeline(argv[2], int(argv[3]) if len(argv) > 3 else 1)
     if len(argv) < 3 or argv[1] not in ("kernels", "counters"):
         raise SystemExit(__doc__)
     (kernels if argv[1] == "kernels" else counters)(argv[2], argv[3:])
