@@ -1030,7 +1030,7 @@ __global__ __launch_bounds__(NT, 4) void fold_slab_kernel(
     typename CT<Real>::T2* __restrict__ bins, const int* __restrict__ a_dev,
     const int* __restrict__ ai_dev, uint32_t mask, int log2B, int log2m, int w, int loops,
     int tiled, int T, int P, int log2sc, typename CT<Real>::T2* __restrict__ yc,
-    const int* __restrict__ offs, int Rl, int log2sigma) {
+    const int* __restrict__ offs, int Rl, int log2sigma, FoldClear clr) {
   using T2 = typename CT<Real>::T2;
   using V2 = typename NV<Real>::type;
   using AC = typename AccOf<Real>::A;
@@ -1141,6 +1141,10 @@ __global__ __launch_bounds__(NT, 4) void fold_slab_kernel(
 
   // (a, ai) of every loop and the comb offsets into LDS: their loads go out before the first
   // chunk's, so that waiting for them does not wait for the chunk
+  // the caller's clear, spread over the grid (one 4-byte store per thread at n = 2^27)
+  for (size_t i = (size_t)blockIdx.x * NT + tid; i < clr.nbm; i += (size_t)gridDim.x * NT)
+    clr.bm[i] = 0u;
+  if (blockIdx.x == 0 && tid < clr.nmax) clr.maxf[tid] = 0xFFFFFFFFu;
   int av = 0, aiv = 0, ov = 0;
   if (tid < loops) {
     av = a_dev[tid];
@@ -1284,7 +1288,8 @@ template <typename Real, int PMAX, int NT>
 static hipError_t slab_launch(const void* x, const void* fT, void* bins, const int* a_dev,
                               const int* ai_dev, uint32_t mask, int log2B, int log2m, int w,
                               int loops, int tiled, int T, int P, int lsc, void* yc,
-                              const int* offs, int Rl, int log2sigma, hipStream_t st) {
+                              const int* offs, int Rl, int log2sigma, const FoldClear& clr,
+                              hipStream_t st) {
   using T2 = typename CT<Real>::T2;
   const unsigned nslabs = 1u << (log2B - lsc);
   const size_t lds = slab_lds_bytes(sizeof(Real) == 4, loops, lsc, P);
@@ -1310,7 +1315,7 @@ static hipError_t slab_launch(const void* x, const void* fT, void* bins, const i
   }
   hipLaunchKernelGGL((fold_slab_kernel<Real, PMAX, NT>), dim3(blocks), dim3(NT), lds, st,
                      (const T2*)x, (const T2*)fT, (T2*)bins, a_dev, ai_dev, mask, log2B, log2m,
-                     w, loops, tiled, T, P, lsc, (T2*)yc, offs, Rl, log2sigma);
+                     w, loops, tiled, T, P, lsc, (T2*)yc, offs, Rl, log2sigma, clr);
   return hipGetLastError();
 }
 
@@ -1319,15 +1324,15 @@ static hipError_t slab_dispatch_nt(int P, const void* x, const void* fT, void* b
                                    const int* a_dev, const int* ai_dev, uint32_t mask, int log2B,
                                    int log2m, int w, int loops, int tiled, int T, int lsc,
                                    void* yc, const int* offs, int Rl, int log2sigma,
-                                   hipStream_t st) {
+                                   const FoldClear& clr, hipStream_t st) {
   if (P <= 16)
     return slab_launch<Real, 16, NT>(x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w, loops,
-                                     tiled, T, P, lsc, yc, offs, Rl, log2sigma, st);
+                                     tiled, T, P, lsc, yc, offs, Rl, log2sigma, clr, st);
   if (P <= 24)
     return slab_launch<Real, 24, NT>(x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w, loops,
-                                     tiled, T, P, lsc, yc, offs, Rl, log2sigma, st);
+                                     tiled, T, P, lsc, yc, offs, Rl, log2sigma, clr, st);
   return slab_launch<Real, 32, NT>(x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w, loops,
-                                   tiled, T, P, lsc, yc, offs, Rl, log2sigma, st);
+                                   tiled, T, P, lsc, yc, offs, Rl, log2sigma, clr, st);
 }
 
 // 512 threads per workgroup: two chunks of loads in flight per workgroup at 4 loads per thread
@@ -1335,9 +1340,10 @@ template <typename Real>
 static hipError_t slab_dispatch(int P, const void* x, const void* fT, void* bins,
                                 const int* a_dev, const int* ai_dev, uint32_t mask, int log2B,
                                 int log2m, int w, int loops, int tiled, int T, int lsc, void* yc,
-                                const int* offs, int Rl, int log2sigma, hipStream_t st) {
+                                const int* offs, int Rl, int log2sigma, const FoldClear& clr,
+                                hipStream_t st) {
   return slab_dispatch_nt<Real, 512>(P, x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w,
-                                     loops, tiled, T, lsc, yc, offs, Rl, log2sigma, st);
+                                     loops, tiled, T, lsc, yc, offs, Rl, log2sigma, clr, st);
 }
 
 int fold_slab_applies(int is_fp32, int n, int B, int w, int loops) {
@@ -1364,7 +1370,8 @@ hipError_t filter_rows_launch(int is_fp32, const void* f, void* fT, int B, int w
 hipError_t fold_columns_launch(int is_fp32, const void* x, const void* fT, void* bins,
                                const int* a_dev, const int* ai_dev, int n, int B, int w,
                                int loops, int tiled, int by_column, int remap, void* yc,
-                               const int* offs, int Rl, int W, hipStream_t st) {
+                               const int* offs, int Rl, int W, const FoldClear& clr,
+                               hipStream_t st) {
   int log2B = 0, log2n = 0, log2W = 0;
   while ((1 << log2B) < B) ++log2B;
   while ((1LL << log2n) < n) ++log2n;
@@ -1377,10 +1384,12 @@ hipError_t fold_columns_launch(int is_fp32, const void* x, const void* fT, void*
     const int lsc = Rl <= 64 ? slab_log2sc(is_fp32, log2B, log2m, loops, P) : -1;
     if (lsc < 0) return hipErrorInvalidValue;
     return is_fp32 ? slab_dispatch<float>(P, x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w,
-                                          loops, tiled, T, lsc, yc, offs, Rl, log2sigma, st)
+                                          loops, tiled, T, lsc, yc, offs, Rl, log2sigma, clr, st)
                    : slab_dispatch<double>(P, x, fT, bins, a_dev, ai_dev, mask, log2B, log2m, w,
-                                           loops, tiled, T, lsc, yc, offs, Rl, log2sigma, st);
+                                           loops, tiled, T, lsc, yc, offs, Rl, log2sigma, clr,
+                                           st);
   }
+  if (clr.nbm || clr.nmax) return hipErrorInvalidValue;   // only the slab fold clears
   // columns per block: one 128-byte line of x (8 complex128 / 16 complex64), fewer if B is
   // smaller
   int log2lw = is_fp32 ? 4 : 3;

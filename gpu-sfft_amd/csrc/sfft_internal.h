@@ -114,16 +114,27 @@ hipError_t perm_filter_launch(int is_fp32, const void* x, const void* f, void* b
 // (perm = ai).  by_column = 0: each thread gathers its taps from global memory (remap:
 // XCD-aware workgroup order) and writes bucket order.  With yc, the comb samples of the Rl
 // offsets offs[] are copied too, column-major: yc[r*W + lo*m + row] =
-// x[off_r + (row*(B/sigma) + lo)*sigma] (m = n/B, sigma = n/W; needs sigma <= B)
+// x[off_r + (row*(B/sigma) + lo)*sigma] (m = n/B, sigma = n/W; needs sigma <= B).  clr: the
+// slab fold (by_column = 1) clears it first; the per-thread fold takes only an empty one.
 int fold_rows_pitch(int B, int w);
 // whether the LDS-staged slab fold takes this shape (at most 64 comb loops)
 int fold_slab_applies(int is_fp32, int n, int B, int w, int loops);
 hipError_t filter_rows_launch(int is_fp32, const void* f, void* fT, int B, int w, int tiled,
                               hipStream_t st);
+// A clear the slab fold does before its first loads: bm[0, nbm) = 0 and maxf[0, nmax) = all
+// ones (a transform's approved-residue bitmaps and largest approved residues, which the comb
+// cutoff's bitmap kernel fills later on the same stream).  Empty: nothing is cleared.
+struct FoldClear {
+  uint32_t* bm = nullptr;
+  size_t nbm = 0;
+  uint32_t* maxf = nullptr;
+  int nmax = 0;
+};
 hipError_t fold_columns_launch(int is_fp32, const void* x, const void* fT, void* bins,
                                const int* a_dev, const int* ai_dev, int n, int B, int w,
                                int loops, int tiled, int by_column, int remap, void* yc,
-                               const int* offs, int Rl, int W, hipStream_t st);
+                               const int* offs, int Rl, int W, const FoldClear& clr,
+                               hipStream_t st);
 // same fold, reading the tap values from the staging array T[l*w + i] of the streaming pass
 // batch: one persistent launch over the permute+filter of several signals
 struct PfJob {

@@ -525,8 +525,9 @@ sfft_status run_front(sfft_plan* P, const void* d_x, const HashArgs& h, hipStrea
 }
 
 // fold: permutation + filtering of every loop into bins_t
-sfft_status run_fold(sfft_plan* P, const void* d_x, hipStream_t st) {
+sfft_status run_fold(sfft_plan* P, const void* d_x, hipStream_t st, const FoldClear* clr) {
   const sfft_params& p = P->p;
+  if (clr && !P->columns) return SFFT_ERR_INVALID;   // only the slab fold clears
   const int tiled = (p.fold_mode == SFFT_FOLD_REFERENCE) && (p.n < (1 << 27));
   if (P->stream)
     HIPCK(fold_taps_launch(P->fp32, P->taps, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, p.loops,
@@ -534,7 +535,7 @@ sfft_status run_fold(sfft_plan* P, const void* d_x, hipStream_t st) {
   else if (P->columns)
     HIPCK(fold_columns_launch(P->fp32, d_x, P->frows, P->bins_t, P->d_a, P->d_ai, p.n, p.B, p.w,
                               p.loops, tiled, P->bins_perm, 1, P->comb_y, P->d_offs,
-                              p.RLFC_loops, p.W_RLFC, st));
+                              p.RLFC_loops, p.W_RLFC, clr ? *clr : FoldClear(), st));
   else
     HIPCK(perm_filter_launch(P->fp32, d_x, P->d_ft, P->bins_t, P->d_ai, p.n, p.B, p.w, p.loops,
                              tiled, st));
@@ -589,11 +590,15 @@ sfft_status run_back(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t
   return v != SFFT_OK ? v : run_locate(P, h, d_nodes, st, marks);
 }
 
-// Column front end: the fold also copies the comb samples, so it runs first; then the comb
-// chain (comb FFT from the copied samples, cutoff, bitmaps) on the plan stream and the bucket
-// chain (bucket FFT, cutoff) on the second stream, joined before the reverse hash.  Profiling
-// runs them serially; its "streaming pass" interval is then the fold (with the comb copy) and
-// its "fold" interval is empty.
+// Column front end: the fold also copies the comb samples, so it runs first (and does the
+// transform's clear of the approved-residue bitmaps on its way: no clear launch); then the
+// comb chain (comb FFT from the copied samples, cutoff, bitmaps) on the plan stream and the
+// bucket chain (bucket FFT, cutoff) on the second stream, joined before the reverse hash.  The
+// comb FFT is enqueued before the bucket chain: in a captured graph it is then the fold's
+// first dependent, which the graph replay keeps on the fold's queue, so the longer chain
+// crosses no queue.  Profiling runs everything serially; its "clear" interval is then empty,
+// its "streaming pass" interval is the fold (with the comb copy) and its "fold" interval is
+// empty.
 static sfft_status run_all_columns(sfft_plan* P, const void* d_x, const HashArgs& h,
                                    Node* d_nodes) {
   sfft_status v;
@@ -601,10 +606,14 @@ static sfft_status run_all_columns(sfft_plan* P, const void* d_x, const HashArgs
   auto mark = [&](int i) -> hipError_t {
     return prof ? hipEventRecord(P->ev[i], P->st) : hipSuccess;
   };
+  FoldClear clr;
+  clr.bm = h.bitmap;
+  clr.nbm = (size_t)h.loops_loc * (size_t)h.bm_words;
+  clr.maxf = (uint32_t*)h.maxfirst;
+  clr.nmax = h.loops_loc;
   HIPCK(mark(0));
-  HIPCK(hash_clear_launch(h, P->st));
   HIPCK(mark(1));
-  if ((v = run_fold(P, d_x, P->st)) != SFFT_OK) return v;
+  if ((v = run_fold(P, d_x, P->st, &clr)) != SFFT_OK) return v;
   HIPCK(mark(2));
   if (prof) {
     HIPCK(fft_run(comb_fft(P, d_x), P->st));
@@ -613,10 +622,10 @@ static sfft_status run_all_columns(sfft_plan* P, const void* d_x, const HashArgs
     return run_locate(P, h, d_nodes, P->st, true);
   }
   HIPCK(hipEventRecord(P->ev_fork, P->st));
+  HIPCK(fft_run(comb_fft(P, d_x), P->st));
   HIPCK(hipStreamWaitEvent(P->st2, P->ev_fork, 0));
   if ((v = run_bucket(P, P->st2, false)) != SFFT_OK) return v;
   HIPCK(hipEventRecord(P->ev_join, P->st2));
-  HIPCK(fft_run(comb_fft(P, d_x), P->st));
   if ((v = run_cut(P, h, P->st, false)) != SFFT_OK) return v;
   HIPCK(hipStreamWaitEvent(P->st, P->ev_join, 0));
   return run_locate(P, h, d_nodes, P->st, false);

@@ -97,7 +97,9 @@ HashArgs hash_args(const sfft_plan* P, int* d_count);
 // profiling events)
 sfft_status run_front(sfft_plan* P, const void* d_x, const HashArgs& h, hipStream_t st,
                       bool marks);
-sfft_status run_fold(sfft_plan* P, const void* d_x, hipStream_t st);
+// clr: cleared by the slab fold first (column front end only; nullptr: nothing)
+sfft_status run_fold(sfft_plan* P, const void* d_x, hipStream_t st,
+                     const FoldClear* clr = nullptr);
 sfft_status run_back(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t st, bool marks);
 sfft_status run_bucket(sfft_plan* P, hipStream_t st, bool marks);
 sfft_status run_locate(sfft_plan* P, const HashArgs& h, Node* d_nodes, hipStream_t st,

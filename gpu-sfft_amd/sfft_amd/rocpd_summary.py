@@ -13,10 +13,11 @@ NAME_PART filters the kernel names (substring match, any of them).
 
     python3 gpu-sfft_amd/sfft_amd/rocpd_summary.py timeline RUN_results.db [LAST]
 
-`timeline`: per transform of a `--kernel-trace` run (from a `hash_clear_kernel` dispatch, the
-first kernel of a transform, to the next `estimate_kernel`, its last), the span, the union of
-kernel busy time and the idle rest; then the last LAST transforms of each precision (default 1)
-kernel by kernel: start, duration, grid, workgroup, LDS bytes and VGPRs per dispatch.
+`timeline`: per transform of a `--kernel-trace` run (from its first kernel, `hash_clear_kernel`
+or, with the column front end, `fold_slab_kernel`, to the next `estimate_kernel`, its last),
+the span, the union of kernel busy time and the idle rest; then the last LAST transforms of
+each precision (default 1) kernel by kernel: start, duration, grid, workgroup, LDS bytes and
+VGPRs per dispatch.
 
     python3 gpu-sfft_amd/sfft_amd/rocpd_summary.py join KERNELS.txt COUNTERS.txt [...]
 
@@ -97,10 +98,11 @@ def counters(path, parts):
 
 
 def _transforms(rows):
-    """(first, last) row indices of each transform: hash_clear_kernel .. estimate_kernel."""
+    """(first, last) row indices of each transform: from its first kernel (hash_clear_kernel,
+    or fold_slab_kernel where the slab fold does the clear) to its estimate_kernel."""
     out, i = [], 0
     while i < len(rows):
-        if "hash_clear_kernel" in rows[i]["name"]:
+        if "hash_clear_kernel" in rows[i]["name"] or "fold_slab_kernel" in rows[i]["name"]:
             j = i
             while j < len(rows) and "estimate_kernel" not in rows[j]["name"]:
                 j += 1

@@ -231,7 +231,9 @@ sfft_status sfft_plan_set_graph(sfft_plan *plan, int on);
 sfft_status sfft_plan_set_profiling(sfft_plan *plan, int on);
 sfft_status sfft_plan_stage_times(sfft_plan *plan, double *seconds7);
 /* raw intervals: clear, streaming pass (0 with gathers), comb FFT + cutoff + bitmaps,
- * permute+filter (or the tap fold), bucket FFT + cutoff, reverse hash, estimation */
+ * permute+filter (or the tap fold), bucket FFT + cutoff, reverse hash, estimation.  With the
+ * column front end the slab fold does the clear and the comb copy: clear and permute+filter
+ * read 0 and the streaming pass interval is the fold. */
 sfft_status sfft_plan_interval_times(sfft_plan *plan, double *seconds7);
 
 /* Debug/parity taps into the last transform (device pointers owned by the plan):
