@@ -280,6 +280,57 @@ __device__ __forceinline__ void fft_stage16(T2* lds, int RP, int r, int log2ncol
   __syncthreads();
 }
 
+// The last radix-4 stage (span ns = R/8) and the radix-2 stage after it (span R/2) in
+// registers: task (column c, jm < R/8) reads the 8 elements jm + i*R/8, runs the radix-4
+// butterflies j = jm (inputs i = 0, 2, 4, 6) and j = jm + R/8 (inputs i = 1, 3, 5, 7, with the
+// same twiddles W_{4ns}^{m*jm}), whose outputs k belong at jm + k*R/8 and R/2 + jm + k*R/8,
+// then the four radix-2 butterflies j' = jm + k*R/8 (twiddle W_R^{j'}) on those pairs, whose
+// outputs belong at j' and j' + R/2.  The two stages' own operations in their own order (the
+// same bits), with one LDS round trip instead of two.  A task writes exactly the 8 elements it
+// read, so it stores each output as soon as it has it (no barrier between its reads and its
+// writes).  So every task runs exactly once: a second thread on the same elements (a clamped
+// duplicate) could read them after their owner has rewritten them.  A thread takes up to
+// EPT/8 tasks (one when EPT < 8; threads past the tasks idle, whole waves at a time).
+template <typename T2, int EPT, int NT, bool TW1>
+__device__ __forceinline__ void fft_stage8(T2* lds, int RP, int r, int log2ncol,
+                                           const T2* __restrict__ tw4,
+                                           const T2* __restrict__ twR, int inverse) {
+  constexpr int TPT = EPT >= 8 ? EPT / 8 : 1;
+  const int tid = threadIdx.x;
+  const int R8 = 1 << (r - 3);
+  const int ntask = (1 << log2ncol) * R8;
+#pragma unroll
+  for (int kk = 0; kk < TPT; ++kk) {
+    const int ti = tid + kk * NT;
+    if (ti < ntask) {
+      const int jm = ti & (R8 - 1);
+      T2* col = lds + (ti >> (r - 3)) * RP;
+      T2 x[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) x[i] = col[LPAD(jm + i * R8)];
+      if (TW1) {
+        const T2* w = tw4 + 4 * ((R8 - 4) / 3 + jm);
+        const T2 w1 = w[0], w2 = w[1], w3 = w[2];
+        x[2] = cmul(x[2], w1);
+        x[4] = cmul(x[4], w2);
+        x[6] = cmul(x[6], w3);
+        x[3] = cmul(x[3], w1);
+        x[5] = cmul(x[5], w2);
+        x[7] = cmul(x[7], w3);
+      }
+      bfly4(x[0], x[2], x[4], x[6], inverse);
+      bfly4(x[1], x[3], x[5], x[7], inverse);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const T2 a1 = cmul(x[2 * k + 1], twR[jm + k * R8]);
+        col[LPAD(jm + k * R8)] = cadd(x[2 * k], a1);
+        col[LPAD(jm + (4 + k) * R8)] = csub(x[2 * k], a1);
+      }
+    }
+  }
+  __syncthreads();
+}
+
 template <typename T2, int EPT, int NT>
 __device__ __forceinline__ void fft_stage2(T2* lds, int RP, int r, int log2ncol, int log2ns,
                                            const T2* __restrict__ twR) {
@@ -484,8 +535,9 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
   // ---- sub-FFT of every column: radix-4 stages, then one radix-2 stage if r is odd
   const T2* __restrict__ twR = (const T2*)A.twR;
   const T2* __restrict__ tw4 = (const T2*)A.tw4;
-  // FP32: pairs of radix-4 stages as radix-16 steps in registers, then a last radix-4 stage
-  // if two bits remain (the same stages in the same order); FP64: radix-4 stages
+  // FP32: pairs of radix-4 stages as radix-16 steps in registers; FP64: radix-4 stages.  Then
+  // the remaining radix-4 stages, the last one fused with the radix-2 stage when r is odd (the
+  // same stages in the same order)
   int log2ns = 0;
   int rem = r;
   while (sizeof(Real) == 4 && rem >= 4) {
@@ -497,6 +549,15 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
     rem -= 4;
   }
   while (rem >= 2) {
+    if (rem == 3) {   // the last radix-4 stage with the radix-2 stage: one LDS round trip
+      if (log2ns > 0)
+        fft_stage8<T2, EPT, NT, true>(lds, RP, r, log2ncol, tw4, twR, A.inverse);
+      else
+        fft_stage8<T2, EPT, NT, false>(lds, RP, r, log2ncol, tw4, twR, A.inverse);
+      log2ns += 3;
+      rem = 0;
+      break;
+    }
     T2 wc[EPT / 4][3];
     if (log2ns > 0) {
       stage4_twiddles<T2, EPT, NT>(wc, r, log2ncol, log2ns, tw4);
