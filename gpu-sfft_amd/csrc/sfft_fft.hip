@@ -142,6 +142,7 @@ struct PassArgs {
   uint16_t* digits; // optional first digits of the keys (same positions)
   int log2nlo, log2m;   // kLoadCombRows: g at in[(g mod 2^log2nlo) << log2m | g >> log2nlo]
   const int* perm;      // kLoadPerm: g of batch b at in[g * perm[b] mod 2^log2N]
+  const int* perm_inv;  // kLoadPerm: the inverse of perm[b] mod 2^log2N
 };
 
 template <typename T2>
@@ -441,7 +442,19 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
   const T2* __restrict__ in = (const T2*)A.in + b * A.in_bstride;
   const T2* __restrict__ cx =
       (LOAD == kLoadComb) ? (const T2*)A.comb_x + A.comb_offs[b] : (const T2*)nullptr;
-  const uint32_t pm = (LOAD == kLoadPerm) ? (uint32_t)A.perm[b] : 0u;
+  // kLoadPerm: tile column c takes the input positions p = q0 + c + nq*X (X < R), so that the
+  // tile's ncol columns read whole lines.  Position p holds element g = p * pinv mod N of the
+  // canonical order (perm[b] * pinv = 1 mod N), which is element v = g / nq of sub-FFT
+  // g mod nq = (q0 + c) * pinv mod nq: the column is that sub-FFT, its elements land at their
+  // own v (the same sub-FFTs and values as loading g at g * perm[b], grouped differently).
+  const uint32_t pinv = (LOAD == kLoadPerm) ? (uint32_t)A.perm_inv[b] : 0u;
+  const uint32_t nmask = (1u << A.log2N) - 1u;
+  const int log2nq = A.log2N - log2Lcur;
+  auto perm_v = [&](int e) __attribute__((always_inline)) {
+    const uint32_t pos = (uint32_t)q0 + (uint32_t)(e & (ncol - 1)) +
+                         ((uint32_t)(e >> log2ncol) << log2nq);
+    return (int)(((pos * pinv) & nmask) >> log2nq);
+  };
   // loads in flight per batch: the plain tiles take batches of 8 bytes x 4 (FP32, 512
   // threads) or 16 bytes x 2 (FP64, 1024 threads: within 64 registers); the comb-gathering
   // pass issues all of its loads together
@@ -462,7 +475,7 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
       else if (LOAD == kLoadCombRows)
         src = in + (((g & ((1LL << A.log2nlo) - 1)) << A.log2m) | (g >> A.log2nlo));
       else if (LOAD == kLoadPerm)
-        src = in + (((uint32_t)g * pm) & ((1u << A.log2N) - 1u));
+        src = in + ((uint32_t)q0 + (uint32_t)c + ((uint32_t)v << log2nq));
       else
         src = in + g;
       vv[k] = *(const V2*)src;
@@ -522,7 +535,7 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
       for (int k = 0; k < LB; ++k) {
         const int e = min(tid + (kb + k) * NT, TE - 1);
         const int c = e & (ncol - 1);
-        const int v = e >> log2ncol;
+        const int v = (LOAD == kLoadPerm) ? perm_v(e) : e >> log2ncol;
         T2 val;
         val.x = vv[k].x;
         val.y = vv[k].y;
@@ -583,7 +596,12 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
       if (first) {
         c = e >> r;
         kk = e & (R - 1);
-        g = (q0 + c) * R + kk;
+        // kLoadPerm: column c is sub-FFT (q0 + c) * pinv mod nq
+        const long long qc =
+            (LOAD == kLoadPerm)
+                ? (long long)(((uint32_t)(q0 + c) * pinv) & (((uint32_t)nq) - 1u))
+                : q0 + c;
+        g = qc * R + kk;
       } else {
         c = e & (ncol - 1);
         kk = e >> log2ncol;
@@ -636,8 +654,9 @@ static hipError_t dispatch_pass(const PassArgs& A, int load, int store, int tw, 
 }
 
 static int pass_log2q(const FftTables& T, const FftLaunch& L, int p, int r, int log2Lprev) {
-  const int qmax = (L.qmax > 0 && L.qmax <= FFT_QMAX) ? L.qmax : FFT_QMAX;
   const int load = (p == 0) ? L.load : kLoadPlain;
+  // a kLoadPerm pass takes as many columns as fit: whole lines of its input per tile
+  const int qmax = (load != kLoadPerm && L.qmax > 0 && L.qmax <= FFT_QMAX) ? L.qmax : FFT_QMAX;
   const int cap = tile_cap(T.is_fp32, load);
   int log2Q = 0;
   while ((1 << (log2Q + 1)) <= qmax && ((1 << (log2Q + 1)) << r) <= cap) ++log2Q;
@@ -711,6 +730,8 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.log2nlo = L.comb_log2nlo;
     A.log2m = L.comb_log2m;
     A.perm = L.perm;
+    A.perm_inv = L.perm_inv;
+    if (load == kLoadPerm && (!L.perm || !L.perm_inv)) return hipErrorInvalidValue;
     A.nbatch = L.batch;
     A.keys_tiled = (last && p > 0 && L.keys_tiled) ? 1 : 0;
     A.digits = last ? L.digits : nullptr;

@@ -78,6 +78,8 @@ struct FftLaunch {
   int comb_log2m = 0;            // in[b*in_bstride + (g mod 2^log2nlo) << log2m | g >> log2nlo]
   const int* perm = nullptr;     // kLoadPerm: element g of batch b at
                                  // in[b*in_bstride + (g * perm[b] mod N)] (perm[b] odd)
+  const int* perm_inv = nullptr; // kLoadPerm: perm_inv[b] * perm[b] = 1 mod N (the first
+                                 // pass reads whole lines of in and regroups by it)
   int qmax = 0;                  // most columns per tile (power of two; 0 = the default)
   int keys_tiled = 0;            // keys in the last pass's tile order (see KeyMap)
   uint16_t* digits = nullptr;    // optional: each key's top 12 bits below the sign bit (the

@@ -564,6 +564,7 @@ sfft_status run_bucket(sfft_plan* P, hipStream_t st, bool marks) {
   if (P->bins_perm) {   // bucket r of loop l at column r * ai_l mod B
     f.load = kLoadPerm;
     f.perm = P->d_ai;
+    f.perm_inv = P->d_a;
   }
   HIPCK(fft_run(f, st));
   // ascending bucket lists: the hit list is ordered by (loop, bucket, walk)
