@@ -211,6 +211,36 @@ def run_precision(args, S, prm, prec, seeds_draws, snr, log):
     return res
 
 
+# the functions whose source text a kernel's counter figures belong to (sfft_fft.hip)
+KERNEL_SOURCE = {"fold_slab_kernel": ("fold_slab_kernel",),
+                 "perm_filter_kernel": ("pf_body", "perm_filter_kernel"),
+                 "stream_scatter_kernel": ("stream_scatter_kernel",)}
+
+
+def kernel_source_digest(kname):
+    """First 16 hex digits of the sha256 of a kernel's source text in sfft_fft.hip: for each
+    function of KERNEL_SOURCE[kname], from the line that opens its template (or its
+    __global__ / __device__ line) to the first closing brace in column 0 after its name.
+    Counter figures profiled for one text are reported as matching only while it is
+    unchanged."""
+    import hashlib
+    with open(os.path.join(ROOT, "gpu-sfft_amd", "csrc", "sfft_fft.hip")) as fh:
+        lines = fh.read().split("\n")
+    heads = ("template", "__global__", "__device__")
+    parts = []
+    for fn in KERNEL_SOURCE.get(kname, (kname,)):
+        at = next(i for i, ln in enumerate(lines) if f" {fn}(" in ln and "void" in ln)
+        start = at
+        while start > 0 and not lines[start].startswith(heads):
+            start -= 1
+        if start > 0 and not lines[start].startswith("template") and lines[start - 1].startswith(
+                "template"):
+            start -= 1
+        end = next(i for i in range(at, len(lines)) if lines[i] == "}")
+        parts.append("\n".join(lines[start:end + 1]))
+    return hashlib.sha256("\n".join(parts).encode()).hexdigest()[:16]
+
+
 def kernel_roofline(prm, prec, front, iv_ms, config):
     """The dominant kernel's rate: algorithmic bytes per launch over its event-timed duration.
     Column slab fold: the signal read once, the filter once, the bins and the comb samples
@@ -241,10 +271,8 @@ def kernel_roofline(prm, prec, front, iv_ms, config):
             entry = json.load(fh).get(f"{kname}_{config}_{prec}")
         if isinstance(entry, dict):
             traffic, tsrc = entry.get("bytes"), entry.get("source")
-            import hashlib
-            with open(os.path.join(ROOT, "gpu-sfft_amd", "csrc", "sfft_fft.hip"), "rb") as fh:
-                sha = hashlib.sha256(fh.read()).hexdigest()[:16]
-            extra["traffic_matches_source"] = entry.get("kernel_source_sha256") == sha
+            extra["traffic_matches_source"] = (
+                entry.get("kernel_source_sha256") == kernel_source_digest(kname))
     out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
            "traffic_source": tsrc, "kernel": kname, "bytes_per_launch": alg,
