@@ -245,8 +245,9 @@ def kernel_roofline(prm, prec, front, iv_ms, config):
     """The dominant kernel's rate: algorithmic bytes per launch over its event-timed duration.
     Column slab fold: the signal read once, the filter once, the bins and the comb samples
     written once.  Gather fold: a gathered tap charged one 64-byte granule (at most the whole
-    signal), the filter once, the bins once.  Stream scatter: the signal once, the tap and comb
-    staging written once."""
+    signal), the filter once, the bins once; beside it the same with the 128-byte line each
+    random gather really fetches (no cap: that is the traffic the counters see).  Stream
+    scatter: the signal once, the tap and comb staging written once."""
     n, B, w, L = prm["n"], prm["B"], prm["w"], prm["loops"]
     e = 8 if prec == "fp32" else 16
     Wc, Rl = prm["W_RLFC"], prm["RLFC_loops"]
@@ -259,7 +260,7 @@ def kernel_roofline(prm, prec, front, iv_ms, config):
         alg = n * e + w * e + L * B * e + Rl * Wc * e
     else:
         alg = min(L * w * GATHER_GRANULE, n * e) + w * e + L * B * e
-        b128 = min(L * w * LINE, n * e) + w * e + L * B * e
+        b128 = L * w * LINE + w * e + L * B * e   # the lines the gathers really fetch
         extra = {"granule_bytes": GATHER_GRANULE,
                  "achieved_granule128": round(b128 / (k_ms * 1e-3) / 1e9, 1),
                  "frac_granule128": round(b128 / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
