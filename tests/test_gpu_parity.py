@@ -250,6 +250,32 @@ def test_transform_deterministic(S):
     assert c1 == c2 and n1.tobytes() == n2.tobytes()
 
 
+def test_draw_arrays_reused_and_rewritten_in_place(S):
+    """The binding keeps the draws struct for the caller's own int32 arrays: the same array
+    objects, rewritten in place with another draw set, give that draw set's transform, and
+    lists (converted afresh on every call) agree with arrays."""
+    prm = params(SMALL_CONFIGS[1])
+    n = prm["n"]
+    x, _, _ = sparse_signal(n, prm["k"], 3)
+    ft, ff = oracle_filter(prm)
+    win = O.filter_window_slice(ff, n, prm["B"])
+    d1 = draws(n, prm["loops"], prm["RLFC_loops"], prm["W_RLFC"], 9)
+    d2 = draws(n, prm["loops"], prm["RLFC_loops"], prm["W_RLFC"], 10)
+    plan = S.Plan(prm, "fp64")
+    plan.set_filter(_dev(ft), win)
+    dx = _dev(x)
+    want1 = plan.execute(dx, *[np.array(v, dtype=np.int32) for v in d1])
+    want2 = plan.execute(dx, *[np.array(v, dtype=np.int32) for v in d2])
+    assert want1[0].tobytes() != want2[0].tobytes()
+    a, ai, offs = [np.array(v, dtype=np.int32) for v in d1]
+    got1 = plan.execute(dx, a, ai, offs)
+    a[:], ai[:], offs[:] = d2
+    got2 = plan.execute(dx, a, ai, offs)
+    got3 = plan.execute(dx, *[list(map(int, v)) for v in d2])
+    for got, want in ((got1, want1), (got2, want2), (got3, want2)):
+        assert got[1] == want[1] and got[0].tobytes() == want[0].tobytes()
+
+
 def test_hit_overflow_is_reported(S):
     prm = params(SMALL_CONFIGS[1])
     n = prm["n"]
