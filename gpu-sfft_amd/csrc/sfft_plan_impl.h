@@ -52,7 +52,7 @@ struct sfft_plan {
   // FFT columns per tile (0 = as many as fit); the bucket FFTs default to 4 columns so that a
   // loops x B batch spreads over enough workgroups to fill the device (the autotuner of the
   // batch executor may change both)
-  int qmax_W = 0, qmax_B = 4;
+  int qmax_W = 0, qmax_B = 16;
   int* adj_head = nullptr;      // adjoint scratch: loops*B list heads, 2*cap links
   int* adj_next = nullptr;
   const void* d_ft = nullptr;
