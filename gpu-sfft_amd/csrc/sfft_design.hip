@@ -200,7 +200,7 @@ extern "C" sfft_status sfft_make_filter_ex(double lobefrac, double tolerance, in
   HCK(hipMalloc(&g, (size_t)n * 16));
   HCK(hipMalloc(&G, (size_t)n * 16));
   HCK(hipMalloc(&scr, (size_t)n * 16));
-  HCK(aux_fold((const double2*)d_filter_time, (double2*)g, w, n, st));       // :336-352 (Q7)
+  HCK(aux_fold((const double2*)d_filter_time, (double2*)g, w, n, st));       // :336-352 (drops f[w-1])
   HCK(fft_plain(g, G, scr, ln, 1, 0, 0, st));                               // FFT_n, :362-365
   hG.resize((size_t)n);
   HCK(hipMemcpyAsync(hG.data(), G, (size_t)n * 16, hipMemcpyDeviceToHost, st));
@@ -210,7 +210,7 @@ extern "C" sfft_status sfft_make_filter_ex(double lobefrac, double tolerance, in
     ff = ff_own.data();
   }
   {
-    // boxcar running sum (cuExperiment.cu:373-396, Q8)
+    // boxcar running sum (cuExperiment.cu:373-396), a sequential recurrence
     double sx = 0.0, sy = 0.0;
     for (int i = 0; i < b_f; ++i) {
       sx += hG[i].x;
@@ -230,7 +230,7 @@ extern "C" sfft_status sfft_make_filter_ex(double lobefrac, double tolerance, in
       ff[i].x /= mx;
       ff[i].y /= mx;
     }
-    // phase ramp by a sequential complex recurrence (cuExperiment.cu:407-421, Q8)
+    // phase ramp by a sequential complex recurrence (cuExperiment.cu:407-421)
     double ox = 1.0, oy = 0.0;
     double stx, sty;
     sincos(-2 * M_PI * (w / 2) / n, &sty, &stx);

@@ -1062,7 +1062,7 @@ __global__ __launch_bounds__(FC_NT) void fold_columns_kernel(
     for (int k = 0; k < FC_CHUNK; ++k)
       if (t0 + k < nmain) acc.add(xv[k], fv[k]);
   }
-  if (tiled && (int)r < residue) {   // the reference's tail tap (Q2): another column
+  if (tiled && (int)r < residue) {   // the reference's tail tap of its tiled fold (n < 2^27): another column
     const uint32_t id = r * (uint32_t)T + B;
     acc.add(x[(id * ai) & mask], frow[T]);
   }
@@ -1327,7 +1327,7 @@ __global__ __launch_bounds__(NT, 4) void fold_slab_kernel(
             }
           }
         }
-        if (tiled && (int)r < residue) {   // the reference's tail tap (Q2): another column
+        if (tiled && (int)r < residue) {   // the reference's tail tap of its tiled fold (n < 2^27): another column
           const uint32_t id = r * (uint32_t)T + B;
           acc.add(x[(id * ai) & mask], fT[(size_t)r * (size_t)P + T]);
         }

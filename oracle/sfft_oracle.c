@@ -71,7 +71,7 @@ void orc_derive_params(int q, int k, double Bcst, int loops_loc, int loops_thres
   o->loops_thresh = loops_thresh;
   o->loops_est = loops_est;
   o->loops = loops_loc + loops_est;
-  o->RLFC_cst = (double)RLFC_cst;        /* parsed with atoi (experiment.c:62, Q12) */
+  o->RLFC_cst = (double)RLFC_cst;        /* parsed with atoi (experiment.c:62) */
   o->RLFC_loops = RLFC_loops;
   o->tolerance = tolerance;
   int n = (int)pow(2, q);
@@ -246,7 +246,7 @@ void orc_make_multiple_t_taps(double lobefrac, double tolerance, int n, int b_f,
     filter_time[j].y = 0.0;                   /* makeDolphchebyshevKernel2 */
   }
   free(v);
-  /* fold around 0: g[0..h) = f[h..2h), g[n-h..n) = f[0..h)  (Q7: f[w-1] dropped) */
+  /* fold around 0: g[0..h) = f[h..2h), g[n-h..n) = f[0..h)  (f[w-1] dropped) */
   orc_c64 *g = (orc_c64 *)calloc((size_t)n, sizeof(orc_c64));
   memcpy(g, filter_time + h, sizeof(orc_c64) * (size_t)h);
   memcpy(g + n - h, filter_time, sizeof(orc_c64) * (size_t)h);
@@ -254,7 +254,7 @@ void orc_make_multiple_t_taps(double lobefrac, double tolerance, int n, int b_f,
   orc_c64 *scr = (orc_c64 *)malloc(sizeof(orc_c64) * (size_t)n);
   orc_fft64(g, ln, 0, scr);                   /* FFT_n forward, cuExperiment.cu:362-365 */
   if (tap_G) memcpy(tap_G, g, sizeof(orc_c64) * (size_t)n);
-  /* boxcar running sum (cuExperiment.cu:373-396, Q8) */
+  /* boxcar running sum (cuExperiment.cu:373-396), a sequential recurrence */
   orc_c64 s = {0.0, 0.0};
   for (int i = 0; i < b_f; ++i) {
     s.x += g[i].x;
@@ -274,7 +274,7 @@ void orc_make_multiple_t_taps(double lobefrac, double tolerance, int n, int b_f,
     filter_freq[i].x /= mx;
     filter_freq[i].y /= mx;
   }
-  /* phase ramp by a sequential complex recurrence (cuExperiment.cu:407-421, Q8) */
+  /* phase ramp by a sequential complex recurrence (cuExperiment.cu:407-421) */
   orc_c64 off = {1.0, 0.0}, step;
   sincos(-2 * M_PI * (w / 2) / n, &step.y, &step.x);
   for (long long i = 0; i < n; ++i) {
@@ -321,12 +321,12 @@ void orc_perm_filter64(const orc_c64 *x, int n, const orc_c64 *f, int w, int B, 
         const uint32_t id = (uint32_t)(t * B + r);
         cmac64(&acc, x[(id * uai) & mask], f[id]);
       }
-      if (r < residue) {                              /* faulty tail thread :744-755 (Q2) */
+      if (r < residue) {                              /* faulty tail thread :744-755 (n < 2^27) */
         const uint32_t id = (uint32_t)(r * tiles + B);
         cmac64(&acc, x[(id * uai) & mask], f[id]);
       }
     } else {
-      for (int i = r; i < w; i += B) {                /* permFilterKernel :766-772 (Q3) */
+      for (int i = r; i < w; i += B) {                /* permFilterKernel :766-772, without its racy += */
         const uint32_t id = (uint32_t)i;
         cmac64(&acc, x[(id * uai) & mask], f[id]);
       }
@@ -392,7 +392,7 @@ void orc_samples32(const orc_c32 *X, int N, float *s) {
 }
 
 /* K-th largest by quickselect on a copy (the reference sorts; the order statistic is the
- * same value), then one ascending pass: s > T, plus the first `need` ties (Q9). */
+ * same value), then one ascending pass: s > T, plus the first `need` ties (lowest indices first). */
 #define DEFINE_CUTOFF(SFX, T)                                                              \
   static T kth_largest_##SFX(T *a, int N, int K) {                                         \
     int lo = 0, hi = N - 1, target = K - 1; /* index in descending order */               \
@@ -466,7 +466,7 @@ static int cmp_pair_first(const void *a, const void *b) {   /* comp_struct3, uti
 }
 
 /* findUpperBound (innerLoop.cu:901-920): lower bound, but the last index when
- * val >= last.first (Q5) */
+ * val >= last.first (findUpperBound's last-index rule) */
 static int find_upper_bound(const orc_pair *P, int n, int val) {
   int low = 0, high = n - 1;
   if (val >= P[high].first) return high;
@@ -635,7 +635,7 @@ int orc_outer_loop64(const orc_plan *p, const orc_c64 *x, const orc_c64 *filter_
     T_pf += now_s() - t;
     t = now_s();
     orc_fft64(row, lB, 0, scr);
-    if (l < p->loops_loc) {   /* estimation loops skip the unused cutoff (Q15) */
+    if (l < p->loops_loc) {   /* estimation loops skip the cutoff, whose result is unused */
       orc_samples64(row, B, s);
       orc_cutoff64(s, B, K, J);
       if (J_out) memcpy(J_out + (size_t)l * K, J, sizeof(int) * (size_t)K);
@@ -735,7 +735,7 @@ void orc_metric_dense(orc_node *nodes, int I_F, int k, const int *large_freq, in
                       const orc_c64 *x_f, int *missed_est, int *missed_res, double *err_total) {
   orc_c64 *hat = (orc_c64 *)calloc((size_t)n, sizeof(orc_c64));
   qsort(nodes, (size_t)I_F, sizeof(orc_node), cmp_node_abs_desc);
-  const int top = k < I_F ? k : I_F;   /* the reference reads k nodes even if I_F < k (Q11) */
+  const int top = k < I_F ? k : I_F;   /* the reference reads k nodes even if I_F < k */
   for (int i = 0; i < top; ++i) hat[nodes[i].key] = nodes[i].value;
   qsort(nodes, (size_t)I_F, sizeof(orc_node), cmp_node_key);
   int found = 0, large_found = 0;
@@ -772,7 +772,7 @@ void orc_metric_dense(orc_node *nodes, int I_F, int k, const int *large_freq, in
  *              (-gi F.y/d, gi F.x/d) for the imaginary part  (d = |F|^2),
  *   g_bins_t[l] = unnormalised inverse FFT_B(u[l])  (the conjugate transpose of FFT_B),
  *   g_x[(i ai_l) mod n] += conj(f[i]) g_bins_t[l][r(i)], loop by loop, taps of the tiled
- *              region (or all taps) first, then the reference fold's tail taps (Q2).
+ *              region (or all taps) first, then the reference fold's tail taps (n < 2^27).
  * pinned by: value_apply == the transform's node values (bit for bit), the adjoint identity
  * <A x, y> = <x, A^T y> on random vectors, and the dense matrix of A on tiny n.
  * ---------------------------------------------------------------------------------------- */

@@ -63,9 +63,9 @@ void orc_dft_any64(orc_c64 *data, int N, int inverse);
 
 /* ---- filter design: make_multiple_t (cuExperiment.cu:248-447) ---------------------------
  * Dolph-Chebyshev window (cuExperiment.cu:279-328, 538-552, 593-601, NaN->0 quirk kept),
- * circular shift (cuShift :507-522), fold with the dropped last tap (:336-352, Q7),
- * sequential boxcar running sum (:373-396, Q8), normalisation (:400-403), sequential
- * phase-ramp recurrence (:407-421, Q8), inverse FFT and /n (:424-447).
+ * circular shift (cuShift :507-522), fold with the dropped last tap f[w-1] (:336-352),
+ * sequential boxcar running sum (:373-396), normalisation (:400-403), sequential
+ * phase-ramp recurrence (:407-421), inverse FFT and /n (:424-447).
  * filter_time[w], filter_freq[n].  pinned by: closed-form Chebyshev samples, brute-force
  * O(w^2) DFT of the window at small w, max|filter_freq| = 1, passband flatness, and the
  * time/frequency pair identity IDFT(filter_freq)[0..w)/n = filter_time. */
@@ -79,23 +79,23 @@ void orc_make_multiple_t_taps(double lobefrac, double tolerance, int n, int b_f,
                               orc_c64 *tap_V, orc_c64 *tap_G);
 
 /* ---- stage functions -------------------------------------------------------------------- */
-/* fold modes: 0 = reference (n < 2^27: tiled fold + faulty tail, innerLoop.cu:725-757, Q2;
- * n >= 2^27: every tap, innerLoop.cu:760-773, Q3 without the race), 1 = every tap. */
+/* fold modes: 0 = reference (n < 2^27: tiled fold + faulty tail, innerLoop.cu:725-757;
+ * n >= 2^27: every tap, innerLoop.cu:760-773, without the race), 1 = every tap. */
 enum { ORC_FOLD_REFERENCE = 0, ORC_FOLD_ALL_TAPS = 1 };
 
 /* bins_t[r] = sum over taps i = r (mod B) of x[(i*ai) & (n-1)] * f[i]  (innerLoop.cu:194-246).
  * pinned by: brute force over exactly-representable integer inputs (order-free sums) in
- * test_oracle_stages, including the Q2 tail at n < 2^27. */
+ * test_oracle_stages, including the tiled fold's tail taps at n < 2^27. */
 void orc_perm_filter64(const orc_c64 *x, int n, const orc_c64 *f, int w, int B, int ai,
                        int fold_mode, orc_c64 *bins_t);
 /* FP32 path: complex64 input/taps, Kahan-compensated per-bucket fold. */
 void orc_perm_filter32(const orc_c32 *x, int n, const orc_c32 *f, int w, int B, int ai,
                        int fold_mode, orc_c32 *bins_t);
 
-/* |X|^2 = fma(re, re, im*im)  (samplesKernel, innerLoop.cu:708-718, Q10). */
+/* |X|^2 = re*re + im*im, each product rounded (samplesKernel, innerLoop.cu:708-718). */
 void orc_samples64(const orc_c64 *X, int N, double *s);
 void orc_samples32(const orc_c32 *X, int N, float *s);
-/* cutoff (innerLoop.cu:551-642): the K indices with largest s, ties by lower index (Q9),
+/* cutoff (innerLoop.cu:551-642): the K indices with largest s, ties by lower index,
  * written ascending.  pinned by: brute-force argsort with (-s, index) keys incl. forced ties. */
 void orc_cutoff64(const double *s, int N, int K, int *out);
 void orc_cutoff32(const float *s, int N, int K, int *out);
@@ -107,16 +107,17 @@ void orc_cutoff32(const float *s, int N, int K, int *out);
 void orc_rlfc64(const orc_c64 *x, int n, int W, int offset, int B_thresh, int *J2_out);
 int orc_merge_J2(int *J2, int count);   /* radix_sort_t + dedup; returns num_RLFC */
 
-/* reverse hash + voting (innerLoop.cu:300-384, 783-817, 901-920; Q5, Q6).
+/* reverse hash + voting (innerLoop.cu:300-384, 783-817, 901-920).
  * J: B_thresh sorted bucket indices.  vote[n] (int32) accumulates; hits appended in the
  * sequential order (J ascending, walk order).  Returns the new hit count, or -1 when the hit
- * list would exceed capacity (Q11).  pinned by: brute-force enumeration of approved
- * positions in each window with the Q5 rule, votes by dictionary (test_oracle_stages). */
+ * list would exceed capacity.  pinned by: brute-force enumeration of approved
+ * positions in each window with findUpperBound's last-index rule, votes by dictionary
+ * (test_oracle_stages). */
 int orc_reverse_hash(const int *J, int B_thresh, int n, int B, int a, int ai, int loop_thresh,
                      int *vote, int *hits, int hit_count, int hit_capacity,
                      const int *J2, int num_RLFC, int W);
 
-/* estimation (innerLoop.cu:396-466, 825-870, 924-937; Q4): lower median over all loops,
+/* estimation (innerLoop.cu:396-466, 825-870, 924-937): lower median over all loops,
  * real and imaginary independently.  filter_freq is read at index (n-d) mod n only; the
  * caller passes a window of m = n/B entries laid out as win[(idx + m/2) mod n] for
  * idx in [n-m/2, n) U [0, m/2), i.e. win[j] = filter_freq[(j - m/2) mod n].

@@ -4,7 +4,7 @@ functions on the same inputs.
 
 restr_loc_freq_coeff draws its comb offset with libc drand48() (innerLoop.cu:122-123): the test
 seeds libc, predicts the draw, and re-seeds before the call.  reverse_hash appends to d_I in the
-order its atomics land (Q14), so hit lists are compared as sets; the vote table d_vote is
+order its atomics land, so hit lists are compared as sets; the vote table d_vote is
 compared entry for entry.
 """
 import ctypes as C

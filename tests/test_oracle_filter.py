@@ -40,7 +40,7 @@ def test_design_stages(cfg):
     ref = windows.chebwin(w, at=-20 * np.log10(tol), sym=True)
     ratio = win_exact / ref
     assert np.max(np.abs(ratio / ratio[h] - 1)) < 1e-6
-    # fold (Q7: f[w-1] dropped) and FFT_n
+    # fold (f[w-1] dropped) and FFT_n
     g = np.zeros(n, dtype=np.complex128)
     g[:h] = shifted[h:2 * h]
     g[n - h:] = shifted[:h]

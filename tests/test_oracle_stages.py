@@ -157,7 +157,7 @@ def test_perm_filter_term_then_add_random_inputs(mode):
 
 def test_estimate_values_rounding_random_bins():
     """Estimation divides as innerLoop.cu:848-852 writes it: |F|^2 = F.x*F.x + F.y*F.y and
-    bins*conj(F) with products rounded and combined, then the lower median (Q4); restated in
+    bins*conj(F) with products rounded and combined, then the lower median; restated in
     numpy on random (non-exact) bins and filter values, bit for bit."""
     rng = np.random.default_rng(12)
     n, B, loops = 1 << 12, 64, 7
@@ -298,7 +298,7 @@ def test_reverse_hash_overflow_reported():
 
 def test_estimate_values_exact_medians():
     """bins chosen as v_l * F_l with |F| = 1 (exact quarter turns) give v_l back exactly; the
-    node value is the lower median of real and imaginary parts separately (Q4)."""
+    node value is the lower median of real and imaginary parts separately."""
     rng = np.random.default_rng(8)
     n, B, loops = 1 << 12, 64, 7
     m = n // B
