@@ -221,9 +221,10 @@ def kernel_source_digest(kname):
     """First 16 hex digits of the sha256 of a kernel's source text in sfft_fft.hip: for each
     function of KERNEL_SOURCE[kname], from the line that opens its template (or its
     __global__ / __device__ line) to the first closing brace in column 0 after its name.
-    Counter figures profiled for one text are reported as matching only while it is
-    unchanged."""
+    Comments and whitespace are left out.  Counter figures profiled for one text are reported
+    as matching only while its code is unchanged."""
     import hashlib
+    import re
     with open(os.path.join(ROOT, "gpu-sfft_amd", "csrc", "sfft_fft.hip")) as fh:
         lines = fh.read().split("\n")
     heads = ("template", "__global__", "__device__")
@@ -238,7 +239,10 @@ def kernel_source_digest(kname):
             start -= 1
         end = next(i for i in range(at, len(lines)) if lines[i] == "}")
         parts.append("\n".join(lines[start:end + 1]))
-    return hashlib.sha256("\n".join(parts).encode()).hexdigest()[:16]
+    # the code only: comments and whitespace do not change what was measured
+    text = re.sub(r"/\*.*?\*/", " ", "\n".join(parts), flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    return hashlib.sha256(" ".join(text.split()).encode()).hexdigest()[:16]
 
 
 def kernel_roofline(prm, prec, front, iv_ms, config):
