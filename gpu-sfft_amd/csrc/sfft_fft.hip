@@ -230,6 +230,34 @@ __device__ __forceinline__ void fft_stage4(T2* lds, int RP, int r, int log2ncol,
 // takes the first stage's outputs m of t = 0..3; twiddles W_{16ns}^{t*(m*ns + jm)}) and writes
 // output m'' of butterfly m to a0*16ns + jm + (m + 4m'')*ns.  These are the two stages' own
 // operations in their own order (the same bits), with one LDS round trip instead of two.
+// the two stages of a radix-16 step on its 16 elements in registers (see fft_stage16)
+template <typename T2, bool TW1>
+__device__ __forceinline__ void stage16_regs(T2 (&x)[16], int jm, int ns,
+                                             const T2* __restrict__ tw4, int inverse) {
+  if (TW1) {
+    const T2* w = tw4 + 4 * ((ns - 4) / 3 + jm);
+    const T2 w1 = w[0], w2 = w[1], w3 = w[2];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      x[t + 4] = cmul(x[t + 4], w1);
+      x[t + 8] = cmul(x[t + 8], w2);
+      x[t + 12] = cmul(x[t + 12], w3);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) bfly4(x[t], x[t + 4], x[t + 8], x[t + 12], inverse);
+  // second stage: butterfly m takes x[4m' ... ] -> inputs m' = t: x[t + 4m]
+  const T2* q2 = tw4 + 4 * ((4 * ns - 4) / 3 + jm);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const T2* w = q2 + 4 * m * ns;
+    x[1 + 4 * m] = cmul(x[1 + 4 * m], w[0]);
+    x[2 + 4 * m] = cmul(x[2 + 4 * m], w[1]);
+    x[3 + 4 * m] = cmul(x[3 + 4 * m], w[2]);
+    bfly4(x[4 * m], x[1 + 4 * m], x[2 + 4 * m], x[3 + 4 * m], inverse);
+  }
+}
+
 template <typename T2, int NT, bool TW1>
 __device__ __forceinline__ void fft_stage16(T2* lds, int RP, int r, int log2ncol, int log2ns,
                                             const T2* __restrict__ tw4, int inverse) {
@@ -247,28 +275,7 @@ __device__ __forceinline__ void fft_stage16(T2* lds, int RP, int r, int log2ncol
     const T2* col = lds + c * RP;
 #pragma unroll
     for (int u = 0; u < 16; ++u) x[u] = col[LPAD(tix + u * R16)];
-    if (TW1) {
-      const T2* w = tw4 + 4 * ((ns - 4) / 3 + jm);
-      const T2 w1 = w[0], w2 = w[1], w3 = w[2];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        x[t + 4] = cmul(x[t + 4], w1);
-        x[t + 8] = cmul(x[t + 8], w2);
-        x[t + 12] = cmul(x[t + 12], w3);
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < 4; ++t) bfly4(x[t], x[t + 4], x[t + 8], x[t + 12], inverse);
-    // second stage: butterfly m takes x[4m' ... ] -> inputs m' = t: x[t + 4m]
-    const T2* q2 = tw4 + 4 * ((4 * ns - 4) / 3 + jm);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const T2* w = q2 + 4 * m * ns;
-      x[1 + 4 * m] = cmul(x[1 + 4 * m], w[0]);
-      x[2 + 4 * m] = cmul(x[2 + 4 * m], w[1]);
-      x[3 + 4 * m] = cmul(x[3 + 4 * m], w[2]);
-      bfly4(x[4 * m], x[1 + 4 * m], x[2 + 4 * m], x[3 + 4 * m], inverse);
-    }
+    stage16_regs<T2, TW1>(x, jm, ns, tw4, inverse);
   }
   __syncthreads();
   if (tid < ntask) {
@@ -460,8 +467,10 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
   // pass issues all of its loads together
   constexpr int LB = (LOAD == kLoadComb) ? EPT : (sizeof(Real) == 8 ? 2 : 4);
   constexpr bool split = (TW == kTwSplit);   // log2Lcur > kMaxPassLog
-#pragma unroll
-  for (int kb = 0; kb < EPT; kb += LB) {
+  const T2* __restrict__ twR = (const T2*)A.twR;
+  const T2* __restrict__ tw4 = (const T2*)A.tw4;
+  // batch kb: loads elements e = tid + (kb + k)*NT (k < LB), with the outer twiddle applied
+  auto load_batch = [&](int kb, T2 (&val)[LB]) __attribute__((always_inline)) {
     V2 vv[LB];
 #pragma unroll
     for (int k = 0; k < LB; ++k) {
@@ -479,6 +488,11 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
       else
         src = in + g;
       vv[k] = *(const V2*)src;
+    }
+#pragma unroll
+    for (int k = 0; k < LB; ++k) {
+      val[k].x = vv[k].x;
+      val[k].y = vv[k].y;
     }
     if constexpr (!first) {
       // outer twiddles W_Lcur^((t0+c)*v): a direct table entry, or the product of a high and
@@ -517,42 +531,76 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
       }
 #pragma unroll
       for (int k = 0; k < LB; ++k) {
+        const int e = min(tid + (kb + k) * NT, TE - 1);
+        T2 w;
+        w.x = tw[k].x;
+        w.y = tw[k].y;
+        const T2 tv = cmul(val[k], w);
+        if ((e >> log2ncol) > 0) val[k] = tv;
+      }
+    }
+  };
+  // A full plain tile (TE == NT*EPT): the thread's elements e = tid + k*NT are the elements
+  // v = j + k*R/EPT of column c = tid mod ncol (j = tid / ncol), the inputs of one task of the
+  // first stage -- FP32 (EPT 16): the radix-16 step of fft_stage16 (its task (c, tix = j));
+  // FP64 (EPT 4): the radix-4 butterfly j of fft_stage4.  The thread runs that task on the
+  // loaded values in registers and writes only its outputs to LDS: the same operations on the
+  // same values as storing the loads and running the stage, with one LDS round trip fewer.
+  constexpr int FR = sizeof(Real) == 4 ? 16 : 4;
+  constexpr bool can_fuse = (LOAD == kLoadPlain || LOAD == kLoadCombRows) && EPT == FR;
+  int log2ns0 = 0;
+  bool fused = false;
+  if constexpr (can_fuse) {
+    if (TE == NT * EPT && (1 << r) >= FR) {
+      fused = true;
+      T2 x[EPT];
+#pragma unroll
+      for (int kb = 0; kb < EPT; kb += LB) {
+        T2 val[LB];
+        load_batch(kb, val);
+#pragma unroll
+        for (int k = 0; k < LB; ++k) x[kb + k] = val[k];
+      }
+      const int j = tid >> log2ncol;
+      T2* col = lds + (tid & (ncol - 1)) * RP;
+      if constexpr (EPT == 16) {
+        stage16_regs<T2, false>(x, 0, 1, tw4, A.inverse);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int m2 = 0; m2 < 4; ++m2) col[LPAD((j << 4) + m + 4 * m2)] = x[m2 + 4 * m];
+        log2ns0 = 4;
+      } else {
+        bfly4(x[0], x[1], x[2], x[3], A.inverse);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) col[LPAD((j << 2) + m)] = x[m];
+        log2ns0 = 2;
+      }
+    }
+  }
+  if (!fused) {
+#pragma unroll
+    for (int kb = 0; kb < EPT; kb += LB) {
+      T2 val[LB];
+      load_batch(kb, val);
+#pragma unroll
+      for (int k = 0; k < LB; ++k) {
         // (a clamped element is stored again with its own value: no branch)
         const int e = min(tid + (kb + k) * NT, TE - 1);
         const int c = e & (ncol - 1);
-        const int v = e >> log2ncol;
-        T2 val, w;
-        val.x = vv[k].x;
-        val.y = vv[k].y;
-        w.x = tw[k].x;
-        w.y = tw[k].y;
-        const T2 tv = cmul(val, w);
-        if (v > 0) val = tv;
-        lds[c * RP + LPAD(v)] = val;
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < LB; ++k) {
-        const int e = min(tid + (kb + k) * NT, TE - 1);
-        const int c = e & (ncol - 1);
         const int v = (LOAD == kLoadPerm) ? perm_v(e) : e >> log2ncol;
-        T2 val;
-        val.x = vv[k].x;
-        val.y = vv[k].y;
-        lds[c * RP + LPAD(v)] = val;
+        lds[c * RP + LPAD(v)] = val[k];
       }
     }
   }
   __syncthreads();
 
   // ---- sub-FFT of every column: radix-4 stages, then one radix-2 stage if r is odd
-  const T2* __restrict__ twR = (const T2*)A.twR;
-  const T2* __restrict__ tw4 = (const T2*)A.tw4;
   // FP32: pairs of radix-4 stages as radix-16 steps in registers; FP64: radix-4 stages.  Then
   // the remaining radix-4 stages, the last one fused with the radix-2 stage when r is odd (the
-  // same stages in the same order)
-  int log2ns = 0;
-  int rem = r;
+  // same stages in the same order), after the stage run at load time on a full tile
+  int log2ns = log2ns0;
+  int rem = r - log2ns0;
   while (sizeof(Real) == 4 && rem >= 4) {
     if (log2ns == 0)
       fft_stage16<T2, NT, false>(lds, RP, r, log2ncol, 0, tw4, A.inverse);
