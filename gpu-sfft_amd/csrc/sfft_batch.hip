@@ -373,7 +373,7 @@ extern "C" sfft_status sfft_batch_tune(sfft_batch* b, const void* const* d_x,
   b->grid = bg;
   if ((v = build_jobs(b, d_x)) != SFFT_OK) return v;
   // 2. FFT columns per tile, bucket FFTs then comb FFTs, one choice for the whole batch
-  const int qs[] = {1, 2, 4, 8, 16};
+  const int qs[] = {1, 2, 4, 8, 16, 32};
   int best_q[2] = {0, 0};
   for (int which = 0; which < 2; ++which) {
     float bq = 1e30f;
@@ -442,7 +442,7 @@ extern "C" sfft_status sfft_batch_tune(sfft_batch* b, const void* const* d_x,
 extern "C" sfft_status sfft_batch_set_shape(sfft_batch* b, int lpt, int nt, int grid,
                                             int qmax_bucket, int qmax_comb) {
   if (!b || lpt < 1 || lpt > 9 || (nt != 128 && nt != 256) || grid < 1 || qmax_bucket < 0 ||
-      qmax_bucket > 16 || qmax_comb < 0 || qmax_comb > 16)
+      qmax_bucket > 32 || qmax_comb < 0 || qmax_comb > 32)
     return SFFT_ERR_INVALID;
   drop_graph(b);
   b->jobs_x.clear();

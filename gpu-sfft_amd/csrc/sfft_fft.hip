@@ -87,7 +87,9 @@ __device__ __forceinline__ void bfly4(T2& a0, T2& a1, T2& a2, T2& a3, int invers
 // ============================================================================================
 // FFT pass
 // ============================================================================================
-constexpr int FFT_QMAX = 16;                 // columns per tile
+// columns per tile: 32, so that the sub-FFTs of 256 points (FP32) or 128 (FP64) and fewer
+// also fill a tile
+constexpr int FFT_QMAX = 32;
 // complex elements per tile (LDS): 8192 complex64 (66 KB) or 4096 complex128 (66 KB) with
 // 1024 threads, so that two workgroups (32 waves) fill a CU and one loads while the other
 // computes (1024 threads rather than 512: the n = 2^27 comb and bucket FFT passes run 3-10%

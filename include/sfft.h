@@ -302,11 +302,11 @@ sfft_status sfft_batch_execute(sfft_batch *b, const void *const *d_x, const sfft
 const int *sfft_batch_counts_device(const sfft_batch *b);
 /* Autotuner: times bit-equivalent launch shapes on this batch (reps runs each) and keeps the
  * fastest: permute+filter loops per thread {1..9} x block {128, 256} x persistent grid
- * {1024, 2048, 4096}, and FFT columns per tile {1..16} of the bucket and comb FFTs.
+ * {1024, 2048, 4096}, and FFT columns per tile {1..32} of the bucket and comb FFTs.
  * choice4 (optional): lpt, block, grid, bucket columns | comb columns << 8. */
 sfft_status sfft_batch_tune(sfft_batch *b, const void *const *d_x, const sfft_draws *draws,
                             int reps, int *choice4);
-/* sets a shape explicitly (lpt 1..9, block 128/256, grid >= 1, columns 0..16, 0 = default) */
+/* sets a shape explicitly (lpt 1..9, block 128/256, grid >= 1, columns 0..32, 0 = default) */
 sfft_status sfft_batch_set_shape(sfft_batch *b, int lpt, int block, int grid, int qmax_bucket,
                                  int qmax_comb);
 

@@ -83,7 +83,8 @@ def test_batch_equals_single_transforms(S, precision):
     _same(b.execute(xs, draws), want)
     # every launch shape the autotuner chooses among computes the same bits
     for shape in [(1, 128, 7, 1, 1), (2, 256, 64, 4, 16), (4, 128, 4096, 16, 2),
-                  (3, 256, 2048, 0, 0), (7, 256, 1024, 0, 0), (9, 128, 2048, 4, 4)]:
+                  (3, 256, 2048, 0, 0), (7, 256, 1024, 0, 0), (9, 128, 2048, 4, 4),
+                  (5, 256, 2048, 32, 32)]:
         b.set_shape(*shape)
         _same(b.execute(xs, draws), want)
     # hipGraph: capture once, replay several times with other device work in between
@@ -111,7 +112,7 @@ def test_batch_autotuner_keeps_results(S):
     choice = b.tune(xs, draws, reps=2)
     assert 1 <= choice["loops_per_thread"] <= 9 and choice["block"] in (128, 256)
     assert choice["grid"] in (1024, 2048, 4096)
-    assert choice["bucket_fft_columns"] in (1, 2, 4, 8, 16)
+    assert choice["bucket_fft_columns"] in (1, 2, 4, 8, 16, 32)
     _same(b.execute(xs, draws), want)
     b.set_graph(True)
     _same(b.execute(xs, draws), want)

@@ -44,13 +44,16 @@ def test_fft64_bitexact(S, log2n, inverse):
 
 
 @pytest.mark.parametrize("log2n", [1, 2, 5, 9, 11, 12, 13, 17, 22])
-def test_fft32_bitexact(S, log2n):
+@pytest.mark.parametrize("inverse", [False, True])
+def test_fft32_bitexact(S, log2n, inverse):
+    # at 2^17 (first pass) and 2^22 (both passes) the tiles are full: their first radix-16
+    # step runs on the loaded values
     n = 1 << log2n
     rng = np.random.default_rng(100 + log2n)
     x = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))).astype(np.complex64)
-    got = S.fft(_dev(x)).cpu().numpy()
+    got = S.fft(_dev(x), inverse=inverse).cpu().numpy()
     for c in range(2):
-        assert np.array_equal(bits(got[c]), bits(O.fft32(x[c])))
+        assert np.array_equal(bits(got[c]), bits(O.fft32(x[c], inverse)))
 
 
 # ---- the whole transform, FP64 plain path -----------------------------------------------------
