@@ -49,9 +49,10 @@ struct sfft_plan {
   double2* win = nullptr;
   Node* nodes = nullptr;
   uint8_t* sel = nullptr;       // median-selected loops of the last transform (2 per hit)
-  // FFT columns per tile (0 = as many as fit, up to 16); the bucket FFTs take 16: full tiles
-  // in their second pass (at n = 2^20, k = 1000, where 4 columns would give four times the
-  // workgroups, both measure the same); the batch autotuner may set both
+  // FFT columns per tile (0 = as many as fit, up to 32); the bucket FFTs take 16: full tiles
+  // in their FP64 second pass (at n = 2^20, k = 1000, where 4 columns would give four times
+  // the workgroups, both measure the same; 32, full FP32 tiles too, measured the same at
+  // n = 2^27, k = 16000); the batch autotuner may set both
   int qmax_W = 0, qmax_B = 16;
   int* adj_head = nullptr;      // adjoint scratch: loops*B list heads, 2*cap links
   int* adj_next = nullptr;
