@@ -83,7 +83,8 @@ hipError_t fft_tables_create(int log2n, int inverse, int is_fp32, FftTables* t) 
       if (log2Lcur <= kMaxPassLog) {
         cnt[p][1] = (size_t)1 << log2Lcur;
       } else {
-        t->s[p] = (log2Lcur + 1) / 2;
+        // low table of at most 256 entries: read at scattered indices, it stays in L1
+        t->s[p] = (log2Lcur + 1) / 2 < 8 ? (log2Lcur + 1) / 2 : 8;
         cnt[p][2] = ((size_t)1 << log2Lcur) >> t->s[p];
         cnt[p][3] = (size_t)1 << t->s[p];
       }

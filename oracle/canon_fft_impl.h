@@ -15,7 +15,8 @@
  *      A = subfft_R(a)
  *      out[q Lcur + t + Lprev k] = A[k]                            k < R
  *    for q < N/Lcur, t < Lprev (self-sorting Stockham with a big radix);
- *  - W_T^e for log2 T <= 11 is the table value tw(e, T); above, with s = ceil(log2 T / 2),
+ *  - W_T^e for log2 T <= 11 is the table value tw(e, T); above, with
+ *    s = min(ceil(log2 T / 2), 8) (a low table of at most 256 entries),
  *    W_T^e = cmul(tw((e >> s) << s, T), tw(e & (2^s - 1), T));
  *  - subfft_R: radix-4 Stockham stages (then one radix-2 stage if r is odd); stage with
  *    span ns and radix rho maps c[u] = a[j + u R/rho], multiplies c[u] (u >= 1, ns > 1) by
@@ -114,7 +115,7 @@ void CF_FN(orc_fft)(CF_CPLX *data, int log2N, int inverse, CF_CPLX *scratch) {
         tdir = (CF_CPLX *)malloc(sizeof(CF_CPLX) * Lcur);
         for (size_t e = 0; e < Lcur; ++e) tdir[e] = CF_FN(cf_tw)(e, Lcur, inverse);
       } else {
-        s = (logLcur + 1) / 2;
+        s = (logLcur + 1) / 2 < 8 ? (logLcur + 1) / 2 : 8;
         const size_t nlo = (size_t)1 << s, nhi = Lcur >> s;
         tlo = (CF_CPLX *)malloc(sizeof(CF_CPLX) * nlo);
         thi = (CF_CPLX *)malloc(sizeof(CF_CPLX) * nhi);
