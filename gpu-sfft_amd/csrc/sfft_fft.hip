@@ -427,9 +427,18 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
   constexpr bool first = (TW == kTwNone);   // log2Lprev == 0
   int log2ncol;
   long long q0 = 0, q = 0, t0 = 0;
+  // kLoadCombRowsG (comb rows with nq = 2^d * nlo sub-FFTs, 0 < d <= log2ncol): sub-FFT q
+  // reads every 2^d-th element of row q mod nlo, interleaved with its siblings q + j*nlo
+  // (j < 2^d).  A tile then takes 2^d sibling groups of ncol/2^d adjacent sub-FFTs, so that
+  // it reads whole lines: tile column c is sub-FFT q0 + qoff(c) (the same sub-FFTs and values,
+  // grouped differently).
+  constexpr bool rows = (LOAD == kLoadCombRows || LOAD == kLoadCombRowsG);
+  int log2cpj = 0;
   if (first) {
     log2ncol = min(A.log2Q, A.log2N - log2Lcur);
-    q0 = (long long)tile << log2ncol;
+    log2cpj = log2ncol;
+    if (LOAD == kLoadCombRowsG) log2cpj = log2ncol - ((A.log2N - log2Lcur) - A.log2nlo);
+    q0 = (long long)tile << log2cpj;
   } else {
     log2ncol = A.log2Q;
     const int log2tpq = log2Lprev - A.log2Q;
@@ -442,6 +451,9 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
   // g = (q + nq*v)*Lprev + t0 + c -- both gbase + c + vstride*v (no per-element branches)
   const long long gbase = first ? q0 : q * Lprev + t0;
   const long long vstride = first ? nq : nq * Lprev;
+  auto qoff = [&](int c) __attribute__((always_inline)) {
+    return (long long)(c & ((1 << log2cpj) - 1)) + ((long long)(c >> log2cpj) << A.log2nlo);
+  };
 
   // ---- load (+ outer twiddle) into LDS, column c at lds[c*RP + v].  Every element load of
   // the thread (and every twiddle load) is issued before any is used: in one basic block,
@@ -483,8 +495,10 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
       const T2* src;
       if (LOAD == kLoadComb)
         src = cx + g * (long long)A.comb_sigma;
-      else if (LOAD == kLoadCombRows)
-        src = in + (((g & ((1LL << A.log2nlo) - 1)) << A.log2m) | (g >> A.log2nlo));
+      else if (rows) {
+        const long long gr = (LOAD == kLoadCombRowsG) ? gbase + qoff(c) + vstride * v : g;
+        src = in + (((gr & ((1LL << A.log2nlo) - 1)) << A.log2m) | (gr >> A.log2nlo));
+      }
       else if (LOAD == kLoadPerm)
         src = in + ((uint32_t)q0 + (uint32_t)c + ((uint32_t)v << log2nq));
       else
@@ -549,7 +563,7 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
   // loaded values in registers and writes only its outputs to LDS: the same operations on the
   // same values as storing the loads and running the stage, with one LDS round trip fewer.
   constexpr int FR = sizeof(Real) == 4 ? 16 : 4;
-  constexpr bool can_fuse = (LOAD == kLoadPlain || LOAD == kLoadCombRows) && EPT == FR;
+  constexpr bool can_fuse = (LOAD == kLoadPlain || rows) && EPT == FR;
   int log2ns0 = 0;
   bool fused = false;
   if constexpr (can_fuse) {
@@ -646,11 +660,11 @@ __global__ __launch_bounds__((Tile<Real, LOAD>::nt),
       if (first) {
         c = e >> r;
         kk = e & (R - 1);
-        // kLoadPerm: column c is sub-FFT (q0 + c) * pinv mod nq
+        // kLoadPerm: column c is sub-FFT (q0 + c) * pinv mod nq; kLoadCombRowsG: q0 + qoff(c)
         const long long qc =
             (LOAD == kLoadPerm)
                 ? (long long)(((uint32_t)(q0 + c) * pinv) & (((uint32_t)nq) - 1u))
-                : q0 + c;
+                : (LOAD == kLoadCombRowsG ? q0 + qoff(c) : q0 + c);
         g = qc * R + kk;
       } else {
         c = e & (ncol - 1);
@@ -692,6 +706,8 @@ static hipError_t dispatch_pass(const PassArgs& A, int load, int store, int tw, 
                                 int batch, hipStream_t st) {
   if (load == kLoadCombRows)
     return dispatch_store<Real, kLoadCombRows, kTwNone>(A, store, tiles, batch, st);
+  if (load == kLoadCombRowsG)
+    return dispatch_store<Real, kLoadCombRowsG, kTwNone>(A, store, tiles, batch, st);
   if (load == kLoadPerm)
     return dispatch_store<Real, kLoadPerm, kTwNone>(A, store, tiles, batch, st);
   if (load == kLoadComb)
@@ -756,13 +772,16 @@ hipError_t fft_run(const FftLaunch& L, hipStream_t st) {
     A.log2N = log2N;
     A.r = r;
     A.log2Lprev = log2Lprev;
-    const int load = (p == 0) ? L.load : kLoadPlain;
+    int load = (p == 0) ? L.load : kLoadPlain;
     const int log2Q = pass_log2q(T, L, p, r, log2Lprev);
     int tiles;
     if (p == 0) {
       const int lnq = log2N - log2Lcur;
       const int lq = log2Q < lnq ? log2Q : lnq;
       tiles = 1 << (lnq - lq);
+      // comb rows holding 2^d sibling sub-FFTs per row (0 < d <= columns per tile): grouped
+      const int d = lnq - L.comb_log2nlo;
+      if (load == kLoadCombRows && d > 0 && d <= lq) load = kLoadCombRowsG;
     } else {
       tiles = (int)((1LL << (log2N - log2Lcur)) << (log2Lprev - log2Q));
     }

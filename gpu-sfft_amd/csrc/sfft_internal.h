@@ -51,7 +51,9 @@ __device__ __forceinline__ unsigned sel_first_digit(unsigned long long key) {
   return (__float_as_uint(f) >> 19) & 0xFFFu;
 }
 
-enum FftLoad { kLoadPlain = 0, kLoadComb = 1, kLoadCombRows = 2, kLoadPerm = 3 };
+// kLoadCombRowsG: kLoadCombRows with sibling sub-FFTs grouped per tile (chosen per pass by
+// fft_run; callers set kLoadCombRows)
+enum FftLoad { kLoadPlain = 0, kLoadComb = 1, kLoadCombRows = 2, kLoadPerm = 3, kLoadCombRowsG = 4 };
 enum FftStore { kStoreCplx = 0, kStoreKeys = 1, kStoreBoth = 2 };
 
 // One canonical FFT of `batch` transforms.  Passes ping-pong between out and scratch so that
