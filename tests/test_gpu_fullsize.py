@@ -1,7 +1,8 @@
 """Parity at the headline sizes: one whole transform at n=2^27 on the GPU against the CPU
 oracle on the same signal, filter and draws, bit for bit (hit list in order, node keys and
-values) -- FP64 at n=2^27, k=16000 and FP32 at the same shapes, plus the FP32 n=2^27,
-k=40000, 30 dB workload.  The signal and the filter are made on the device (both are
+values) -- FP64 at n=2^27, k=16000 and FP32 at the same shapes, plus the n=2^27, k=40000,
+30 dB workload in both precisions (its comb FFT of 2^24 points takes three passes, the first
+reading comb rows that interleave two sub-FFTs each).  The signal and the filter are made on the device (both are
 bit-exact with the oracle's own restatements, see test_filter_design_bitexact) and copied to
 the host for the oracle.  Each case takes seconds on the CPU side."""
 import numpy as np
@@ -16,6 +17,7 @@ CASES = {
     "n27_k16000_fp64": (27, 16000, 0.65, 2, 2, 7, 4096, 2, 0.0, "fp64"),
     "n27_k16000_fp32": (27, 16000, 0.65, 2, 2, 7, 4096, 2, 0.0, "fp32"),
     "n27_k40000_30dB_fp32": (27, 40000, 0.7, 3, 2, 8, 32768, 2, 30.0, "fp32"),
+    "n27_k40000_30dB_fp64": (27, 40000, 0.7, 3, 2, 8, 32768, 2, 30.0, "fp64"),
 }
 
 
