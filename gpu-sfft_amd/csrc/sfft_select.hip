@@ -288,21 +288,47 @@ __global__ __launch_bounds__(SEL_NT) void sel_split_kernel(const K* __restrict__
   int* ce = eidx + rb;
   const int begin = blockIdx.x * chunk, end = min(N, begin + chunk);
   const uint16_t* dp = digits ? digits + (long long)seg * seg_stride : nullptr;
-  for (int base = begin; base < end; base += SEL_NT) {
-    const int i = base + tid;
-    const bool ok = i < end;
-    // with the first digits at hand, a key is read only when it is a candidate
-    const unsigned d = dp ? (ok ? (unsigned)dp[i] : 0u) : top_digit<K>(ok ? kp[i] : (K)0);
-    const bool isc = ok && d == d0;
-    const K k = isc ? kp[i] : (K)0;
-    const int pc = wave_append(&s_nc, isc);
-    if (isc) {
-      ck[pc] = k;
-      ci[pc] = (int)key_natural(km, i);
+  // SU elements per thread at a time: their digit (or key) loads are in flight together, then
+  // the candidates' key loads, so that the loop waits twice per SU elements, not twice per
+  // element (the order of a block's candidates and winners is set by the wave appends either
+  // way; every later pass is independent of it)
+  constexpr int SU = 8;
+  for (int base = begin; base < end; base += SEL_NT * SU) {
+    unsigned d[SU];
+    K k[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const int i = base + u * SEL_NT + tid;
+      const bool ok = i < end;
+      if (dp) {
+        d[u] = ok ? (unsigned)dp[i] : 0u;
+      } else {
+        k[u] = ok ? kp[i] : (K)0;
+        d[u] = top_digit<K>(k[u]);
+      }
     }
-    const bool ise = ok && d > d0;
-    const int pe = wave_append(&s_ne, ise);
-    if (emit && ise) ce[pe] = (int)key_natural(km, i);
+    // with the first digits at hand, a key is read only when it is a candidate
+    if (dp) {
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+        const int i = base + u * SEL_NT + tid;
+        k[u] = (i < end && d[u] == d0) ? kp[i] : (K)0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const int i = base + u * SEL_NT + tid;
+      const bool ok = i < end;
+      const bool isc = ok && d[u] == d0;
+      const int pc = wave_append(&s_nc, isc);
+      if (isc) {
+        ck[pc] = k[u];
+        ci[pc] = (int)key_natural(km, i);
+      }
+      const bool ise = ok && d[u] > d0;
+      const int pe = wave_append(&s_ne, ise);
+      if (emit && ise) ce[pe] = (int)key_natural(km, i);
+    }
   }
   __syncthreads();
   if (tid == 0) {
